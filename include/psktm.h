@@ -1,0 +1,280 @@
+/*
+ * psktm.h - C ABI of the MI355X-native prestack Kirchhoff time migration (PSTM).
+ *
+ * Two layers live behind this header, both with C linkage in libpsktm.so:
+ *
+ *  1. The reference's own C API for the migration path, same names and
+ *     argument meaning, so an existing caller relinks against libpsktm.so:
+ *       initTwoAxes / initThreeAxes          src/psktmFunctions.c:1,19
+ *       ktmIBM32toIEEE, fileSizeFourBytes, binReadUints, binReadFloats,
+ *       fileWrite, errorMessage, loadInputData, load*Config
+ *                                            src/ioFunctions.c:4-175
+ *       ktmMigrationGPU                      src/psktmCUDA.cu:211-221
+ *     The reference never defines its structs or IBM masks; the layouts
+ *     below are this library's definition (fields inferred from their uses).
+ *
+ *  2. The extended psktm_* API: device pointers (HBM-resident inputs), an
+ *     explicit HIP stream (passed as void*), status codes, and the options the
+ *     reference lacks (linear interpolation, aperture, 3-D velocity, FP16/BF16
+ *     trace storage, accumulate mode, the adjoint, the rho filter, FP64 CG
+ *     reductions and batched gathers).
+ *
+ * Units: seconds and metres. The image time axis is one-way vertical time tau
+ * (the DSR time is 2*tau at zero offset, src/psktmCPU.c:78-79); offsets are
+ * half-offset vectors (off = (rec - src)/2, scripts/ktmConvertXYtoOffMid.m:12);
+ * the reference's "slowness" argument holds velocity in m/s
+ * (src/psktmCPU.c:48). Layouts: traces[k*tracePts + n] (trace-major),
+ * image[(i*Lj + j)*Ll + l] (x slowest, tau fastest) (src/psktmCPU.c:84-85).
+ *
+ * Errors: legacy entries return void like the reference, leave the image
+ * zeroed on failure and record the reason for psktm_last_error(); extended
+ * entries return psktm_status. The library never prints and never tears down
+ * the device context.
+ */
+#ifndef PSKTM_H
+#define PSKTM_H
+
+#include <stdio.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Reference data structures (undefined in the reference).                  */
+/* ------------------------------------------------------------------------ */
+
+/* One image axis: imagePnts samples at imageStart + i*imageStep
+ * (src/psktmFunctions.c:8-12, src/psktmCUDA.cu:226-237). */
+struct imageGrid {
+    int   imagePnts;
+    float imageStart;
+    float imageStep;
+};
+
+/* Job scalars: trace count, samples per trace, sample interval in seconds
+ * (src/psktmCPU.c:29,59,80,82; src/psktmCUDA.cu:245-247). */
+struct jobParams {
+    int   traceNo;
+    int   tracePts;
+    float traceSampleRate;
+};
+
+/* Open input files of the on-disk job (src/ioFunctions.c:94-153). */
+struct fileInputPtrs {
+    FILE *inputData;
+    FILE *inputSrcX;
+    FILE *inputSrcY;
+    FILE *inputRecX;
+    FILE *inputRecY;
+    int   flag;          /* 1 if any file failed to open */
+};
+
+/* IBM System/360 single: bit 31 sign, bits 30:24 base-16 exponent (bias 64),
+ * bits 23:0 fraction (src/ioFunctions.c:7-9). */
+#define SIGN_MASK     0x80000000u
+#define EXPONENT_MASK 0x7F000000u
+#define FRACTION_MASK 0x00FFFFFFu
+
+/* ------------------------------------------------------------------------ */
+/* Reference API (legacy entry points).                                     */
+/* ------------------------------------------------------------------------ */
+
+/* axis[i] = start + (float)i * step, binary32, no contraction.
+ * src/psktmFunctions.c:1-17 */
+void initTwoAxes(struct imageGrid *imageY, struct imageGrid *imageZ,
+                 float *imageYY, float *imageZZ);
+/* src/psktmFunctions.c:19-39 */
+void initThreeAxes(struct imageGrid *imageX, struct imageGrid *imageY,
+                   struct imageGrid *imageZ, float *imageXX, float *imageYY,
+                   float *imageZZ);
+
+/* IBM -> IEEE with the reference's arithmetic: sign * (fraction / 0xFFFFFF)
+ * * 16^(exp-64) (src/ioFunctions.c:4-40). Divides by 2^24-1, so results are
+ * about 1 ulp high; psktm_ibm32_to_ieee_exact gives the true value. */
+void ktmIBM32toIEEE(unsigned int *input, float *output, int n);
+/* Number of 4-byte words in an open file; rewinds it (src/ioFunctions.c:44-56). */
+int fileSizeFourBytes(FILE *fin);
+/* malloc'd buffer of n words read from fin; rewinds it; caller frees
+ * (src/ioFunctions.c:60-68, 70-78). */
+unsigned int *binReadUints(FILE *fin, int n);
+float *binReadFloats(FILE *fin, int n);
+/* Writes t floats (src/ioFunctions.c:80-84). */
+void fileWrite(FILE *fout, float *ptr, int t);
+/* Records "Can't open <str>." as the last error; the reference printf's it
+ * (src/ioFunctions.c:88-91). */
+void errorMessage(char *str);
+/* Opens ../data/{inputData,inputSrcX,inputSrcY,inputRecX,inputRecY}.bin;
+ * flag = 1 if any fails (src/ioFunctions.c:94-153). Files stay open. */
+void loadInputData(struct fileInputPtrs *input);
+/* Empty in the reference (src/ioFunctions.c:155-175); kept as no-ops.
+ * psktm_load_ktm_config reads inp.txt for real. */
+void loadRTM2dConfig(void);
+void loadKTMConfig(void);
+void loadKDMConfig(void);
+
+/* Drop-in for src/psktmCUDA.cu:211-221 (and for ktmMigrationCPU,
+ * src/psktmCPU.c:13-23, which has the identical list): host arrays in, image
+ * zeroed then overwritten, synchronous. Runs the reference-exact HIP form
+ * (truncating index, IEEE sqrt/div, no contraction, traces summed in k order
+ * per image sample), which is bit-identical to the single-thread reference
+ * loop. No aperture (the reference has none). On any failure the image stays
+ * zeroed and psktm_last_error() says why. */
+void ktmMigrationGPU(struct imageGrid *imageX, struct imageGrid *imageY,
+                     struct imageGrid *imageZ, struct jobParams *config,
+                     float *midX, float *midY, float *offX, float *offY,
+                     float *traces, float *slowness, float *image);
+
+/* ------------------------------------------------------------------------ */
+/* Extended API.                                                            */
+/* ------------------------------------------------------------------------ */
+
+typedef enum {
+    PSKTM_OK = 0,
+    PSKTM_E_INVAL = 1,        /* bad argument or size */
+    PSKTM_E_NODEV = 2,        /* no usable gfx950 device */
+    PSKTM_E_HIP = 3,          /* HIP runtime error */
+    PSKTM_E_ALLOC = 4,        /* device allocation failed */
+    PSKTM_E_LAUNCH = 5,       /* kernel launch failed */
+    PSKTM_E_IO = 6,           /* file I/O */
+    PSKTM_E_UNSUPPORTED = 7   /* valid but not implemented combination */
+} psktm_status;
+
+typedef enum { PSKTM_INTERP_TRUNC = 0, PSKTM_INTERP_LINEAR = 1 } psktm_interp;
+/* IEEE: reference arithmetic (correctly rounded sqrt and divide, no FMA);
+ * FAST: hardware v_sqrt_f32, reciprocal multiply, FMA. */
+typedef enum { PSKTM_MATH_IEEE = 0, PSKTM_MATH_FAST = 1 } psktm_math;
+typedef enum { PSKTM_F32 = 0, PSKTM_F16 = 1, PSKTM_BF16 = 2 } psktm_dtype;
+typedef enum { PSKTM_VEL_1D = 0, PSKTM_VEL_3D = 1 } psktm_vel_kind;
+
+/* Output image axes (x slowest, tau fastest). tau is one-way time. */
+typedef struct psktm_grid {
+    int   nx, ny, nt;
+    float x0, dx, y0, dy, tau0, dtau;
+} psktm_grid;
+
+/* Acquisition: device arrays of length n_traces (half offsets). */
+typedef struct psktm_survey {
+    int          n_traces;
+    int          n_samples;   /* samples per trace */
+    float        dt;          /* sample interval, s */
+    const float *mid_x, *mid_y, *off_x, *off_y;
+} psktm_survey;
+
+/* Trace samples on the device, trace-major [n_traces][n_samples]. For F16 and
+ * BF16 the sample value is stored[k][n] * scale[k]. */
+typedef struct psktm_traces {
+    int          dtype;       /* psktm_dtype */
+    const void  *data;
+    const float *scale;       /* [n_traces]; NULL for F32 */
+} psktm_traces;
+
+/* RMS velocity in m/s on the device: VEL_1D -> v[nt] (laterally invariant, as
+ * in the reference); VEL_3D -> v[(i*ny + j)*nt + l] on the image grid. */
+typedef struct psktm_velocity {
+    int          kind;        /* psktm_vel_kind */
+    const float *v;
+} psktm_velocity;
+
+typedef struct psktm_options {
+    int   interp;             /* psktm_interp */
+    int   math;               /* psktm_math */
+    float aperture;           /* metres, midpoint-to-bin; <= 0 means none */
+    int   tile_x, tile_y;     /* bins per tile; 0 = library choice */
+    int   traces_per_stage;   /* traces staged in LDS at once; 0 = choice */
+} psktm_options;
+
+/* Last error text of the calling thread ("" if none). */
+const char *psktm_last_error(void);
+/* Library version string. */
+const char *psktm_version(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int psktm_device_count(void);
+
+/* A plan holds what depends only on the survey geometry, the image grid and
+ * the options: midpoints binned into tiles and, per output tile, the ordered
+ * list of traces inside its aperture. Build once, migrate many times
+ * (FP32/FP16 forms, every CG iteration). Geometry arrays are read during
+ * creation only. */
+typedef struct psktm_plan psktm_plan;
+psktm_status psktm_plan_create(psktm_plan **plan, const psktm_grid *grid,
+                               const psktm_survey *survey,
+                               const psktm_options *opt, void *stream);
+psktm_status psktm_plan_destroy(psktm_plan *plan);
+/* Total updates (trace, in-aperture bin, tau) one migration performs. */
+psktm_status psktm_plan_updates(const psktm_plan *plan, double *updates);
+/* Bytes of device memory the plan holds. */
+psktm_status psktm_plan_bytes(const psktm_plan *plan, size_t *bytes);
+
+/* Migration (traces -> image), stream-ordered, device pointers. The image is
+ * overwritten unless accumulate != 0. */
+psktm_status psktm_migrate(psktm_plan *plan, const psktm_traces *traces,
+                           const psktm_velocity *vel, float *image,
+                           int accumulate, void *stream);
+/* Adjoint of psktm_migrate (image -> traces, Kirchhoff demigration) with the
+ * same discrete traveltime, interpolation, aperture and weight. traces_out is
+ * FP32 [n_traces][n_samples], overwritten unless accumulate != 0. */
+psktm_status psktm_demigrate(psktm_plan *plan, const float *image,
+                             const psktm_velocity *vel, float *traces_out,
+                             int accumulate, void *stream);
+
+/* Reference-exact migration on device pointers (the kernel behind
+ * ktmMigrationGPU): TRUNC + IEEE, 1-D velocity v[nt], no aperture, FP32. */
+psktm_status psktm_migrate_exact(const psktm_grid *grid,
+                                 const psktm_survey *survey,
+                                 const float *traces, const float *vel,
+                                 float *image, void *stream);
+
+/* Rho filter (scripts/ktmApplyRhoFilter.m:1-64) on device traces
+ * [n_traces][n_samples]: zero-pad to M = 2^ceil(log2 m), multiply the
+ * spectrum by |w| (opt 1), sqrt|w| e^{i pi/4 sgn w} (opt 2) or i w (opt 3)
+ * with w_k = 2 pi k / (dw_us * 1e-6) (no 1/M, as in the reference), keep the
+ * first m samples. out_dtype F32 writes float; F16/BF16 write the quantised
+ * value and scale_out[k] = max|y_k| / qmax (per-trace max-abs scale).
+ * matlab_trace_shift != 0 reproduces the reference's 2-D fftshift, which for
+ * an odd trace count returns trace (c+1) mod n in row c. */
+psktm_status psktm_rho_filter(const float *in, int n_traces, int n_samples,
+                              float dw_us, int opt, int out_dtype, void *out,
+                              float *scale_out, int matlab_trace_shift,
+                              void *stream);
+/* FP32 -> F16/BF16 with per-trace max-abs scales, no filtering. */
+psktm_status psktm_quantize(const float *in, int n_traces, int n_samples,
+                            int out_dtype, void *out, float *scale_out,
+                            void *stream);
+
+/* True IBM decode (fraction / 2^24), bit-exact where the value is a normal
+ * float; flushes below FLT_MIN to signed zero and saturates to +-inf above. */
+void psktm_ibm32_to_ieee_exact(const unsigned int *input, float *output, int n);
+/* Same decode on the device (n words, stream-ordered). */
+psktm_status psktm_ibm32_to_ieee_device(const unsigned int *input,
+                                        float *output, size_t n, void *stream);
+
+/* Reads inp.txt (12 floats: tracePts, dt, traceNo, xStart, xStep, xPnts,
+ * yStart, yStep, yPnts, zStart, zStep, zPnts; written by
+ * scripts/ktmWriteTraceBlockConstOffset.m:154-167, read by
+ * scripts/ktmVisualizeOutputFile.m:22-48) into the structs. */
+psktm_status psktm_load_ktm_config(const char *path, struct imageGrid *imageX,
+                                   struct imageGrid *imageY,
+                                   struct imageGrid *imageZ,
+                                   struct jobParams *config);
+
+/* FP64 reductions for conjugate gradients, stream-ordered, no host sync:
+ * result[0] = sum_i a[i]*b[i] accumulated in double. */
+psktm_status psktm_dot_f64(const float *a, const float *b, size_t n,
+                           double *result, void *stream);
+/* y[i] += (float)(num[0]/den[0]) * sign * x[i], scalars read on the device. */
+psktm_status psktm_axpy_ratio(float *y, const float *x, size_t n,
+                              const double *num, const double *den,
+                              float sign, void *stream);
+/* p[i] = r[i] + (float)(num[0]/den[0]) * p[i]. */
+psktm_status psktm_xpby_ratio(float *p, const float *r, size_t n,
+                              const double *num, const double *den,
+                              void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSKTM_H */

@@ -1,0 +1,23 @@
+"""psktm: MI355X-native prestack Kirchhoff time migration (host side).
+
+The Python layer mirrors the reference's C interface for the migration path
+(src/psktmCPU.c:13-23, src/psktmCUDA.cu:211-221, src/psktmFunctions.c,
+src/ioFunctions.c) over the C ABI of ``libpsktm.so``; every compute call runs
+a HIP kernel on the current device. Torch tensors are used for device memory
+and streams only.
+"""
+from ._lib import (BF16, F16, F32, INTERP_LINEAR, INTERP_TRUNC, MATH_FAST, MATH_IEEE,
+                   VEL_1D, VEL_3D, ImageGrid, JobParams, PsktmError, last_error, load)
+from .api import (Grid, Plan, Survey, demigrate, dot_f64, ibm32_to_ieee,
+                  ibm32_to_ieee_device, ibm32_to_ieee_exact, init_three_axes,
+                  init_two_axes, ktm_migration_gpu, load_ktm_config, migrate,
+                  migrate_exact, quantize, rho_filter)
+
+__all__ = [
+    "BF16", "F16", "F32", "INTERP_LINEAR", "INTERP_TRUNC", "MATH_FAST", "MATH_IEEE",
+    "VEL_1D", "VEL_3D", "ImageGrid", "JobParams", "PsktmError", "last_error", "load",
+    "Grid", "Plan", "Survey", "demigrate", "dot_f64", "ibm32_to_ieee",
+    "ibm32_to_ieee_device", "ibm32_to_ieee_exact", "init_three_axes", "init_two_axes",
+    "ktm_migration_gpu", "load_ktm_config", "migrate", "migrate_exact", "quantize",
+    "rho_filter",
+]

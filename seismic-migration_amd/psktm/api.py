@@ -1,0 +1,341 @@
+"""Host-side API over libpsktm.so.
+
+Two families, as in the C header:
+
+* reference-named entry points on host numpy arrays: :func:`ktm_migration_gpu`
+  (src/psktmCUDA.cu:211-221), :func:`init_three_axes` / :func:`init_two_axes`
+  (src/psktmFunctions.c:1-39), :func:`ibm32_to_ieee` (src/ioFunctions.c:4-40);
+* extended entry points on device tensors (HBM-resident, stream-ordered):
+  :class:`Plan`, :func:`migrate`, :func:`demigrate`, :func:`migrate_exact`,
+  :func:`rho_filter`, :func:`quantize`, :func:`dot_f64`.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (F32, INTERP_LINEAR, MATH_FAST, VEL_1D, VEL_3D, ImageGrid, JobParams,
+                   PsktmGrid, PsktmOptions, PsktmSurvey, PsktmTraces, PsktmVelocity, check)
+
+_p = ctypes.c_void_p
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass(frozen=True)
+class Grid:
+    """Output image axes (x slowest, tau fastest; tau is one-way time)."""
+    nx: int
+    x0: float
+    dx: float
+    ny: int
+    y0: float
+    dy: float
+    nt: int
+    tau0: float
+    dtau: float
+
+    def c(self) -> PsktmGrid:
+        return PsktmGrid(self.nx, self.ny, self.nt, self.x0, self.dx, self.y0, self.dy,
+                         self.tau0, self.dtau)
+
+    def image_grids(self):
+        return (ImageGrid(self.nx, self.x0, self.dx), ImageGrid(self.ny, self.y0, self.dy),
+                ImageGrid(self.nt, self.tau0, self.dtau))
+
+    @property
+    def shape(self):
+        return (self.nx, self.ny, self.nt)
+
+
+def _ptr(t) -> int:
+    return t.data_ptr() if t is not None else 0
+
+
+def _need(t, dtype, name, numel=None):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} must have {numel} elements, got {t.numel()}")
+
+
+def _stream(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return _p(stream.cuda_stream)
+
+
+class Survey:
+    """Acquisition geometry on the device: midpoints and half offsets."""
+
+    def __init__(self, mid_x, mid_y, off_x, off_y, n_samples: int, dt: float):
+        torch = _torch()
+        n = mid_x.numel()
+        for name, t in (("mid_x", mid_x), ("mid_y", mid_y), ("off_x", off_x), ("off_y", off_y)):
+            _need(t, torch.float32, name, n)
+        self.mid_x, self.mid_y, self.off_x, self.off_y = mid_x, mid_y, off_x, off_y
+        self.n_traces, self.n_samples, self.dt = int(n), int(n_samples), float(dt)
+
+    def c(self) -> PsktmSurvey:
+        return PsktmSurvey(self.n_traces, self.n_samples, self.dt, _ptr(self.mid_x),
+                           _ptr(self.mid_y), _ptr(self.off_x), _ptr(self.off_y))
+
+
+def _velocity(vel, grid: Grid):
+    torch = _torch()
+    _need(vel, torch.float32, "vel")
+    if vel.numel() == grid.nt:
+        return PsktmVelocity(VEL_1D, _ptr(vel))
+    if vel.numel() == grid.nx * grid.ny * grid.nt:
+        return PsktmVelocity(VEL_3D, _ptr(vel))
+    raise ValueError(f"velocity must have nt={grid.nt} or nx*ny*nt elements, got {vel.numel()}")
+
+
+def migrate_exact(grid: Grid, survey: Survey, traces, vel, image=None, stream=None):
+    """Reference-exact migration (TRUNC + IEEE, 1-D velocity, no aperture).
+
+    Bit-identical to the single-thread reference loop of src/psktmCPU.c.
+    """
+    torch = _torch()
+    L = _lib.load()
+    _need(traces, torch.float32, "traces", survey.n_traces * survey.n_samples)
+    _need(vel, torch.float32, "vel", grid.nt)
+    if image is None:
+        image = torch.empty(grid.shape, dtype=torch.float32, device=traces.device)
+    _need(image, torch.float32, "image", grid.nx * grid.ny * grid.nt)
+    g, s = grid.c(), survey.c()
+    check(L.psktm_migrate_exact(ctypes.byref(g), ctypes.byref(s), _p(_ptr(traces)),
+                                _p(_ptr(vel)), _p(_ptr(image)), _stream(stream)))
+    return image
+
+
+class Plan:
+    """psktm_plan: traces binned into output tiles with per-tile aperture lists."""
+
+    def __init__(self, grid: Grid, survey: Survey, *, interp=INTERP_LINEAR, math=MATH_FAST,
+                 aperture=0.0, tile_x=0, tile_y=0, traces_per_stage=0, stream=None):
+        L = _lib.load()
+        self.grid, self.survey = grid, survey
+        self.opt = PsktmOptions(int(interp), int(math), float(aperture), int(tile_x),
+                                int(tile_y), int(traces_per_stage))
+        self._h = _p()
+        g, s = grid.c(), survey.c()
+        check(L.psktm_plan_create(ctypes.byref(self._h), ctypes.byref(g), ctypes.byref(s),
+                                  ctypes.byref(self.opt), _stream(stream)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def updates(self) -> float:
+        v = ctypes.c_double()
+        check(_lib.load().psktm_plan_updates(self._h, ctypes.byref(v)))
+        return v.value
+
+    def nbytes(self) -> int:
+        v = ctypes.c_size_t()
+        check(_lib.load().psktm_plan_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._h:
+            _lib.load().psktm_plan_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def migrate(plan: Plan, traces, vel, image=None, *, scale=None, dtype=None, accumulate=False,
+            stream=None):
+    """Migration traces -> image on the plan's grid (device tensors).
+
+    ``traces`` is float32, float16 or bfloat16 [n_traces, n_samples]; for the
+    16-bit forms ``scale`` [n_traces] float32 gives the per-trace scale.
+    """
+    torch = _torch()
+    L = _lib.load()
+    s, g = plan.survey, plan.grid
+    kinds = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
+    if traces.dtype not in kinds:
+        raise TypeError(f"unsupported trace dtype {traces.dtype}")
+    _need(traces, traces.dtype, "traces", s.n_traces * s.n_samples)
+    kind = kinds[traces.dtype]
+    if kind != _lib.F32:
+        _need(scale, torch.float32, "scale", s.n_traces)
+    tr = PsktmTraces(kind, _ptr(traces), _ptr(scale) if kind != _lib.F32 else 0)
+    v = _velocity(vel, g)
+    if image is None:
+        image = torch.empty(g.shape, dtype=torch.float32, device=traces.device)
+    _need(image, torch.float32, "image", g.nx * g.ny * g.nt)
+    check(L.psktm_migrate(plan.handle, ctypes.byref(tr), ctypes.byref(v), _p(_ptr(image)),
+                          int(bool(accumulate)), _stream(stream)))
+    return image
+
+
+def demigrate(plan: Plan, image, vel, traces=None, *, accumulate=False, stream=None):
+    """Adjoint of :func:`migrate`: image -> float32 traces [n_traces, n_samples]."""
+    torch = _torch()
+    L = _lib.load()
+    s, g = plan.survey, plan.grid
+    _need(image, torch.float32, "image", g.nx * g.ny * g.nt)
+    v = _velocity(vel, g)
+    if traces is None:
+        traces = torch.empty((s.n_traces, s.n_samples), dtype=torch.float32, device=image.device)
+    _need(traces, torch.float32, "traces", s.n_traces * s.n_samples)
+    check(L.psktm_demigrate(plan.handle, _p(_ptr(image)), ctypes.byref(v), _p(_ptr(traces)),
+                            int(bool(accumulate)), _stream(stream)))
+    return traces
+
+
+def rho_filter(traces, dw_us: float, opt: int = 2, out_dtype=None, *, matlab_trace_shift=False,
+               stream=None):
+    """Rho filter (scripts/ktmApplyRhoFilter.m) on device traces [n, m].
+
+    Returns float32 traces, or (quantised traces, per-trace scale) when
+    ``out_dtype`` is torch.float16 / torch.bfloat16.
+    """
+    torch = _torch()
+    L = _lib.load()
+    _need(traces, torch.float32, "traces")
+    n, m = traces.shape
+    out_dtype = torch.float32 if out_dtype is None else out_dtype
+    kinds = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
+    out = torch.empty((n, m), dtype=out_dtype, device=traces.device)
+    scale = torch.empty((n,), dtype=torch.float32, device=traces.device) \
+        if out_dtype != torch.float32 else None
+    check(L.psktm_rho_filter(_p(_ptr(traces)), int(n), int(m), float(dw_us), int(opt),
+                             kinds[out_dtype], _p(_ptr(out)), _p(_ptr(scale)),
+                             int(bool(matlab_trace_shift)), _stream(stream)))
+    return out if scale is None else (out, scale)
+
+
+def quantize(traces, out_dtype, *, stream=None):
+    """FP32 -> FP16/BF16 with per-trace max-abs scales."""
+    torch = _torch()
+    L = _lib.load()
+    _need(traces, torch.float32, "traces")
+    n, m = traces.shape
+    kinds = {torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
+    out = torch.empty((n, m), dtype=out_dtype, device=traces.device)
+    scale = torch.empty((n,), dtype=torch.float32, device=traces.device)
+    check(L.psktm_quantize(_p(_ptr(traces)), int(n), int(m), kinds[out_dtype], _p(_ptr(out)),
+                           _p(_ptr(scale)), _stream(stream)))
+    return out, scale
+
+
+def dot_f64(a, b, out=None, *, stream=None):
+    """sum(a*b) of float32 device vectors accumulated in FP64 (device result)."""
+    torch = _torch()
+    L = _lib.load()
+    _need(a, torch.float32, "a")
+    _need(b, torch.float32, "b", a.numel())
+    if out is None:
+        out = torch.empty((1,), dtype=torch.float64, device=a.device)
+    check(L.psktm_dot_f64(_p(_ptr(a)), _p(_ptr(b)), ctypes.c_size_t(a.numel()),
+                          _p(_ptr(out)), _stream(stream)))
+    return out
+
+
+# --------------------------------------------------------------------------
+# Reference-named host entry points.
+# --------------------------------------------------------------------------
+
+def _host(a, dtype=np.float32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def ktm_migration_gpu(imageX: ImageGrid, imageY: ImageGrid, imageZ: ImageGrid,
+                      config: JobParams, midX, midY, offX, offY, traces, slowness,
+                      image=None):
+    """ktmMigrationGPU (src/psktmCUDA.cu:211-221) on host arrays.
+
+    Returns the image (zeroed when the call failed; see :func:`last_error`).
+    """
+    L = _lib.load()
+    arrs = [_host(a) for a in (midX, midY, offX, offY, traces, slowness)]
+    n = imageX.imagePnts * imageY.imagePnts * imageZ.imagePnts
+    if image is None:
+        image = np.empty(n, dtype=np.float32)
+    L.ktmMigrationGPU(ctypes.byref(imageX), ctypes.byref(imageY), ctypes.byref(imageZ),
+                      ctypes.byref(config), *[a.ctypes.data_as(_p) for a in arrs],
+                      image.ctypes.data_as(_p))
+    return image
+
+
+def init_three_axes(imageX: ImageGrid, imageY: ImageGrid, imageZ: ImageGrid):
+    """initThreeAxes (src/psktmFunctions.c:19-39)."""
+    L = _lib.load()
+    xx = np.empty(imageX.imagePnts, np.float32)
+    yy = np.empty(imageY.imagePnts, np.float32)
+    zz = np.empty(imageZ.imagePnts, np.float32)
+    L.initThreeAxes(ctypes.byref(imageX), ctypes.byref(imageY), ctypes.byref(imageZ),
+                    xx.ctypes.data_as(_p), yy.ctypes.data_as(_p), zz.ctypes.data_as(_p))
+    return xx, yy, zz
+
+
+def init_two_axes(imageY: ImageGrid, imageZ: ImageGrid):
+    """initTwoAxes (src/psktmFunctions.c:1-17)."""
+    L = _lib.load()
+    yy = np.empty(imageY.imagePnts, np.float32)
+    zz = np.empty(imageZ.imagePnts, np.float32)
+    L.initTwoAxes(ctypes.byref(imageY), ctypes.byref(imageZ), yy.ctypes.data_as(_p),
+                  zz.ctypes.data_as(_p))
+    return yy, zz
+
+
+def ibm32_to_ieee(words):
+    """ktmIBM32toIEEE (src/ioFunctions.c:4-40), reference arithmetic."""
+    L = _lib.load()
+    w = _host(words, np.uint32)
+    out = np.empty(w.shape, np.float32)
+    L.ktmIBM32toIEEE(w.ctypes.data_as(_p), out.ctypes.data_as(_p), int(w.size))
+    return out
+
+
+def ibm32_to_ieee_exact(words):
+    """True IBM decode (fraction / 2^24) on the host."""
+    L = _lib.load()
+    w = _host(words, np.uint32)
+    out = np.empty(w.shape, np.float32)
+    L.psktm_ibm32_to_ieee_exact(w.ctypes.data_as(_p), out.ctypes.data_as(_p), int(w.size))
+    return out
+
+
+def ibm32_to_ieee_device(words, out=None, *, stream=None):
+    """True IBM decode on the device: int32/uint32 words tensor -> float32."""
+    torch = _torch()
+    L = _lib.load()
+    if words.dtype not in (torch.int32, torch.uint32):
+        raise TypeError("words must be int32 or uint32")
+    _need(words, words.dtype, "words")
+    if out is None:
+        out = torch.empty(words.shape, dtype=torch.float32, device=words.device)
+    _need(out, torch.float32, "out", words.numel())
+    check(L.psktm_ibm32_to_ieee_device(_p(_ptr(words)), _p(_ptr(out)),
+                                       ctypes.c_size_t(words.numel()), _stream(stream)))
+    return out
+
+
+def load_ktm_config(path: str):
+    """Read inp.txt into (imageX, imageY, imageZ, config)."""
+    L = _lib.load()
+    X, Y, Z, J = ImageGrid(), ImageGrid(), ImageGrid(), JobParams()
+    check(L.psktm_load_ktm_config(path.encode(), ctypes.byref(X), ctypes.byref(Y),
+                                  ctypes.byref(Z), ctypes.byref(J)))
+    return X, Y, Z, J
