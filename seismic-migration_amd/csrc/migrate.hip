@@ -1,0 +1,564 @@
+// Fast PSTM kernels for gfx950: forward migration (traces -> image) and its
+// adjoint (image -> traces), both driven by a psktm_plan.
+//
+// Forward (gather, output-stationary). A workgroup owns one output tile of
+// TB bins and NTH*TL tau samples; every lane keeps TB*TL image samples in
+// FP32 registers for the whole tile, so the image is written once. The tile's
+// trace list is streamed in stages of G traces:
+//   fill    - each trace is expanded into LDS as (d[n], d[n+1]-d[n]) pairs in
+//             FP32 (16-bit traces are decoded and scaled here), and lanes
+//             (g, b) tabulate the distance terms A = |s-x|^2, B = |r-x|^2 and
+//             the binary32 aperture bit of every (trace, bin);
+//   compute - per (trace, bin) that passed (a wave-uniform scalar branch),
+//             each lane evaluates its TL samples:
+//               x  = sqrt(tau'^2 + A s') + sqrt(tau'^2 + B s')   (= t/dt)
+//               d  = fma(fract(x), dd[n], d[n]),  n = (int)min(x, ns)
+//               acc += d * sqrt(x)
+//             with s' = 1/(v dt)^2 and tau' = tau/dt; the weight
+//             t*sqrt(tau^2/t) = tau*sqrt(dt)*sqrt(x) (src/psktmCPU.c:84-85)
+//             is completed by one multiply per sample at the end.
+// Out-of-range samples need no test: x is clamped to ns and the pair table's
+// entries outside the valid range of the interpolation rule are zero
+// (TRUNC: 0 < n < ns, src/psktmCPU.c:82; LINEAR: 0 <= n, n+1 < ns).
+// The path is bound by VALU issue (three v_sqrt_f32 and ~9 other VALU ops
+// per update); HBM traffic is the trace rows, read once per tile.
+//
+// Adjoint (scatter, trace-stationary). A workgroup owns a batch of traces
+// from one midpoint cell, accumulates them in LDS with ds_add_f32, visits
+// every tile within reach of the cell and writes each trace once. It uses the
+// same traveltime, clamp, interpolation weights and aperture bits as the
+// forward, so the pair is an exact adjoint up to FP32 rounding.
+#include "internal.hpp"
+
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace psktm {
+namespace {
+
+constexpr int kGMax = 8;      // traces per forward stage (upper bound)
+constexpr int kAdjMax = 8;    // traces per adjoint work item (upper bound)
+constexpr int kTBMax = 16;
+
+struct FwdArgs {
+    int nx, ny, nt, ns;
+    float x0, dx, y0, dy, tau0, dtau;
+    int tile_x, tile_y, tiles_y, n_chunks;
+    float nsf, r2, wscale;
+    int use_ap, all_traces, ntr, interp, dtype, G, nsp, accumulate;
+    const float4 *geo;
+    const int *list;
+    const long long *tile_off;
+    const int *tile_order;
+    const void *traces;
+    const float *scale;
+    const float *slow;
+    const float *tsq;
+    float *image;
+};
+
+struct AdjArgs {
+    int nx, ny, nt, ns;
+    float x0, dx, y0, dy, tau0, dtau;
+    int tile_x, tile_y, tiles_x, tiles_y, n_chunks;
+    int cells_y, cell_mx, cell_my, win_x, win_y;
+    float nsf, r2, wscale;
+    int use_ap, interp, nsa;
+    const float4 *geo;
+    const int *sorted_ids;
+    const int *cell_start;
+    const int *items;
+    const float *slow;
+    const float *tsq;
+    const float *image;
+    float *traces;
+};
+
+__device__ __forceinline__ float load_sample(const void *tr, int dtype, size_t idx, float sc)
+{
+    if (dtype == PSKTM_F32) return static_cast<const float *>(tr)[idx];
+    if (dtype == PSKTM_F16)
+        return __half2float(static_cast<const __half *>(tr)[idx]) * sc;
+    unsigned u = static_cast<const unsigned short *>(tr)[idx];
+    return __uint_as_float(u << 16) * sc;
+}
+
+// LDS pair n of a trace: (d[n], d[n+1]-d[n]) for LINEAR, (d[n], 0) for TRUNC,
+// (0, 0) outside the rule's valid range (n in [0, ns]).
+__device__ __forceinline__ float2 make_pair(const void *tr, int dtype, size_t row, int n,
+                                            int ns, int interp, float sc)
+{
+    if (interp == PSKTM_INTERP_LINEAR) {
+        if (n <= ns - 2) {
+            float d0 = load_sample(tr, dtype, row + n, sc);
+            float d1 = load_sample(tr, dtype, row + n + 1, sc);
+            return make_float2(d0, d1 - d0);
+        }
+        return make_float2(0.f, 0.f);
+    }
+    if (n >= 1 && n <= ns - 1) return make_float2(load_sample(tr, dtype, row + n, sc), 0.f);
+    return make_float2(0.f, 0.f);
+}
+
+// Distance terms and aperture bit of (trace geometry ge, bin (i, j)).
+__device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, float dx,
+                                           float y0, float dy, int use_ap, float r2,
+                                           float2 *ab)
+{
+    const float x = axis_f32(x0, dx, i), y = axis_f32(y0, dy, j);
+    const bool m = !use_ap || in_aperture_f32(ge.x, ge.y, x, y, r2);
+    const float ex = ge.x - x, ey = ge.y - y;
+    const float ax = ex - ge.z, ay = ey - ge.w, bx = ex + ge.z, by = ey + ge.w;
+    *ab = make_float2(fmaf(ax, ax, ay * ay), fmaf(bx, bx, by * by));
+    return m;
+}
+
+template <int NTH, int TB, int TL, bool V3>
+__global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
+{
+    extern __shared__ float2 s_pairs[];
+    __shared__ float2 s_ab[kGMax][TB];
+    __shared__ unsigned s_mask[kGMax];
+
+    const int tid = threadIdx.x;
+    const int T = a.tile_order[blockIdx.x / a.n_chunks];
+    const int chunk = blockIdx.x % a.n_chunks;
+    const int TI = T / a.tiles_y, TJ = T % a.tiles_y;
+    const int i0 = TI * a.tile_x, j0 = TJ * a.tile_y;
+    const int lbase = chunk * (NTH * TL) + tid;
+
+    float tq[TL];
+    float sv[V3 ? TB : 1][TL];
+    float acc[TB][TL];
+#pragma unroll
+    for (int j = 0; j < TL; ++j) {
+        const int l = lbase + j * NTH;
+        tq[j] = (l < a.nt) ? a.tsq[l] : 0.f;
+        if (!V3) sv[0][j] = (l < a.nt) ? a.slow[l] : 0.f;
+    }
+#pragma unroll
+    for (int b = 0; b < TB; ++b) {
+        const int i = i0 + b / a.tile_y, jj = j0 + b % a.tile_y;
+        const bool bin_ok = (b / a.tile_y) < a.tile_x && i < a.nx && jj < a.ny;
+#pragma unroll
+        for (int j = 0; j < TL; ++j) {
+            acc[b][j] = 0.f;
+            if (V3) {
+                const int l = lbase + j * NTH;
+                sv[V3 ? b : 0][j] = (bin_ok && l < a.nt)
+                    ? a.slow[((size_t)i * a.ny + jj) * a.nt + l] : 0.f;
+            }
+        }
+    }
+
+    const long long s0 = a.all_traces ? 0 : a.tile_off[T];
+    const long long s1 = a.all_traces ? a.ntr : a.tile_off[T + 1];
+    const int G = a.G;
+    for (long long s = s0; s < s1; s += G) {
+        // ---- fill: distance terms + aperture bits, then trace pairs -------
+        if (tid < G * TB) {
+            const int g = tid / TB, b = tid % TB;
+            const long long q = s + g;
+            float2 ab = make_float2(0.f, 0.f);
+            bool m = false;
+            if (q < s1) {
+                const int k = a.all_traces ? (int)q : a.list[q];
+                const int bi = b / a.tile_y, bj = b % a.tile_y;
+                const int i = i0 + bi, jj = j0 + bj;
+                if (bi < a.tile_x && i < a.nx && jj < a.ny)
+                    m = pair_terms(a.geo[k], i, jj, a.x0, a.dx, a.y0, a.dy, a.use_ap, a.r2, &ab);
+            }
+            s_ab[g][b] = ab;
+            const unsigned long long bal = __ballot(m);
+            if ((tid % TB) == 0) {
+                const int shift = (tid & 63) - ((tid & 63) % TB);
+                s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
+            }
+        }
+        for (int g = 0; g < G; ++g) {
+            const long long q = s + g;
+            if (q >= s1) break;
+            const int k = a.all_traces ? (int)q : a.list[q];
+            const size_t row = (size_t)k * a.ns;
+            const float sc = (a.dtype == PSKTM_F32) ? 1.f : a.scale[k];
+            float2 *P = s_pairs + g * a.nsp;
+            for (int n = tid; n <= a.ns; n += NTH)
+                P[n] = make_pair(a.traces, a.dtype, row, n, a.ns, a.interp, sc);
+        }
+        __syncthreads();
+        // ---- compute --------------------------------------------------------
+        for (int g = 0; g < G; ++g) {
+            const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
+            if (!m || s + g >= s1) continue;
+            const float2 *P = s_pairs + g * a.nsp;
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                if (!((m >> b) & 1u)) continue;
+                const float2 ab = s_ab[g][b];
+#pragma unroll
+                for (int j = 0; j < TL; ++j) {
+                    const float sl = sv[V3 ? b : 0][j];
+                    const float ua = fmaf(ab.x, sl, tq[j]);
+                    const float ub = fmaf(ab.y, sl, tq[j]);
+                    float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
+                    x = fminf(x, a.nsf);
+                    const int n = (int)x;
+                    const float f = __builtin_amdgcn_fractf(x);
+                    const float2 pr = P[n];
+                    const float d = fmaf(f, pr.y, pr.x);
+                    acc[b][j] = fmaf(d, __builtin_amdgcn_sqrtf(x), acc[b][j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- epilogue: weight tau*sqrt(dt), one write per sample ---------------
+#pragma unroll
+    for (int b = 0; b < TB; ++b) {
+        const int bi = b / a.tile_y, bj = b % a.tile_y;
+        const int i = i0 + bi, jj = j0 + bj;
+        if (bi >= a.tile_x || i >= a.nx || jj >= a.ny) continue;
+#pragma unroll
+        for (int j = 0; j < TL; ++j) {
+            const int l = lbase + j * NTH;
+            if (l >= a.nt) continue;
+            const float w = axis_f32(a.tau0, a.dtau, l) * a.wscale;
+            const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
+            const float v = acc[b][j] * w;
+            a.image[idx] = a.accumulate ? a.image[idx] + v : v;
+        }
+    }
+}
+
+template <int NTH, int TB, int TL, bool V3>
+__global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
+{
+    extern __shared__ float s_acc[];
+    __shared__ float2 s_ab[kAdjMax][TB];
+    __shared__ unsigned s_mask[kAdjMax];
+    __shared__ int s_k[kAdjMax];
+
+    const int tid = threadIdx.x;
+    const int c = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
+    const int p1 = min(p0 + kAdjMax, a.cell_start[c + 1]);
+    const int ng = p1 - p0;
+    for (int q = tid; q < kAdjMax * a.nsa; q += NTH) s_acc[q] = 0.f;
+    if (tid < kAdjMax) s_k[tid] = (tid < ng) ? a.sorted_ids[p0 + tid] : -1;
+    __syncthreads();
+
+    const int CI = c / a.cells_y, CJ = c % a.cells_y;
+    const int ti0 = max(0, CI - a.cell_mx - a.win_x), ti1 = min(a.tiles_x - 1, CI - a.cell_mx + a.win_x);
+    const int tj0 = max(0, CJ - a.cell_my - a.win_y), tj1 = min(a.tiles_y - 1, CJ - a.cell_my + a.win_y);
+    const int lo = (a.interp == PSKTM_INTERP_LINEAR) ? 0 : 1;
+    const int hi = (a.interp == PSKTM_INTERP_LINEAR) ? a.ns - 2 : a.ns - 1;
+    const bool linear = a.interp == PSKTM_INTERP_LINEAR;
+
+    for (int TI = ti0; TI <= ti1; ++TI) {
+        for (int TJ = tj0; TJ <= tj1; ++TJ) {
+            const int i0 = TI * a.tile_x, j0 = TJ * a.tile_y;
+            if (tid < kAdjMax * TB) {
+                const int g = tid / TB, b = tid % TB;
+                float2 ab = make_float2(0.f, 0.f);
+                bool m = false;
+                const int bi = b / a.tile_y, bj = b % a.tile_y;
+                const int i = i0 + bi, jj = j0 + bj;
+                if (g < ng && bi < a.tile_x && i < a.nx && jj < a.ny)
+                    m = pair_terms(a.geo[s_k[g]], i, jj, a.x0, a.dx, a.y0, a.dy, a.use_ap, a.r2, &ab);
+                s_ab[g][b] = ab;
+                const unsigned long long bal = __ballot(m);
+                if ((tid % TB) == 0) {
+                    const int shift = (tid & 63) - ((tid & 63) % TB);
+                    s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
+                }
+            }
+            __syncthreads();
+            unsigned any = 0;
+            for (int g = 0; g < kAdjMax; ++g) any |= s_mask[g];
+            any = __builtin_amdgcn_readfirstlane(any);
+            if (any) {
+                for (int ch = 0; ch < a.n_chunks; ++ch) {
+                    const int lbase = ch * (NTH * TL) + tid;
+                    float tq[TL];
+#pragma unroll
+                    for (int j = 0; j < TL; ++j) {
+                        const int l = lbase + j * NTH;
+                        tq[j] = (l < a.nt) ? a.tsq[l] : 0.f;
+                    }
+#pragma unroll
+                    for (int b = 0; b < TB; ++b) {
+                        if (!((any >> b) & 1u)) continue;
+                        const int i = i0 + b / a.tile_y, jj = j0 + b % a.tile_y;
+                        float im[TL], sl[TL];
+#pragma unroll
+                        for (int j = 0; j < TL; ++j) {
+                            const int l = lbase + j * NTH;
+                            const bool ok = l < a.nt;
+                            const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
+                            im[j] = ok ? a.image[idx] * (axis_f32(a.tau0, a.dtau, l) * a.wscale) : 0.f;
+                            sl[j] = ok ? (V3 ? a.slow[idx] : a.slow[l]) : 0.f;
+                        }
+                        for (int g = 0; g < kAdjMax; ++g) {
+                            const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
+                            if (!((m >> b) & 1u)) continue;
+                            const float2 ab = s_ab[g][b];
+                            float *acc = s_acc + g * a.nsa;
+#pragma unroll
+                            for (int j = 0; j < TL; ++j) {
+                                const float ua = fmaf(ab.x, sl[j], tq[j]);
+                                const float ub = fmaf(ab.y, sl[j], tq[j]);
+                                float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
+                                x = fminf(x, a.nsf);
+                                int n = (int)x;
+                                const float f = __builtin_amdgcn_fractf(x);
+                                const float w = __builtin_amdgcn_sqrtf(x) * im[j];
+                                n = (n >= lo && n <= hi) ? n : a.ns;   // a.ns, a.ns+1: discarded slots
+                                if (linear) {
+                                    const float w1 = w * f;
+                                    atomicAdd(acc + n, w - w1);
+                                    atomicAdd(acc + n + 1, w1);
+                                } else {
+                                    atomicAdd(acc + n, w);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int g = 0; g < ng; ++g) {
+        float *out = a.traces + (size_t)s_k[g] * a.ns;
+        const float *acc = s_acc + g * a.nsa;
+        for (int n = tid; n < a.ns; n += NTH) out[n] += acc[n];
+    }
+}
+
+__global__ void tables_kernel(int nt, float tau0, float dtau, float inv_dt, float inv_dt2,
+                              const float *__restrict__ v, size_t nv, float *__restrict__ slow,
+                              float *__restrict__ tsq)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nv) {
+        const float vv = v[q];
+        slow[q] = inv_dt2 / (vv * vv);
+    }
+    if (q < (size_t)nt) {
+        const float t = axis_f32(tau0, dtau, (int)q) * inv_dt;
+        tsq[q] = t * t;
+    }
+}
+
+__global__ void zero_kernel(float *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// ---- kernel selection ------------------------------------------------------
+
+using FwdFn = void (*)(FwdArgs);
+using AdjFn = void (*)(AdjArgs);
+
+template <int NTH, int TB, bool V3>
+FwdFn fwd_by_tl(int tl)
+{
+    switch (tl) {
+    case 2: return migrate_fwd_kernel<NTH, TB, 2, V3>;
+    case 3: return migrate_fwd_kernel<NTH, TB, 3, V3>;
+    case 4: return migrate_fwd_kernel<NTH, TB, 4, V3>;
+    case 6: return migrate_fwd_kernel<NTH, TB, 6, V3>;
+    case 8: return migrate_fwd_kernel<NTH, TB, 8, V3>;
+    default: return nullptr;
+    }
+}
+
+FwdFn fwd_kernel(int nth, int tb, int tl, bool v3)
+{
+    if (nth == 256) {
+        if (tb == 8) return v3 ? fwd_by_tl<256, 8, true>(tl) : fwd_by_tl<256, 8, false>(tl);
+        return v3 ? fwd_by_tl<256, 16, true>(tl) : fwd_by_tl<256, 16, false>(tl);
+    }
+    if (tb == 8) return v3 ? fwd_by_tl<512, 8, true>(tl) : fwd_by_tl<512, 8, false>(tl);
+    return v3 ? fwd_by_tl<512, 16, true>(tl) : fwd_by_tl<512, 16, false>(tl);
+}
+
+// Register budget per lane for accumulators (+ 3-D slowness): TB*TL*(1+V3).
+constexpr int kRegBudget = 96;
+constexpr int kTLSet[] = {2, 3, 4, 6, 8};
+
+struct FwdShape {
+    int nth, tl, chunks;
+};
+
+FwdShape choose_shape(int nt, int tb, bool v3)
+{
+    const int tlmax = std::min(8, kRegBudget / (tb * (v3 ? 2 : 1)));
+    for (int nth : {256, 512}) {
+        const int need = (nt + nth - 1) / nth;
+        for (int tl : kTLSet)
+            if (tl >= need && tl <= tlmax) return {nth, tl, 1};
+    }
+    int tl = 2;
+    for (int t : kTLSet)
+        if (t <= tlmax) tl = t;
+    return {512, tl, (nt + 512 * tl - 1) / (512 * tl)};
+}
+
+} // namespace
+
+psktm_status prepare_tables(psktm_plan *p, const psktm_velocity &vel, hipStream_t st)
+{
+    const psktm_grid &g = p->grid;
+    const size_t nv = (vel.kind == PSKTM_VEL_3D) ? (size_t)g.nx * g.ny * g.nt : (size_t)g.nt;
+    if (p->slow_cap < nv) {
+        if (p->slow) { (void)hipFree(p->slow); p->bytes -= p->slow_cap * sizeof(float); p->slow = nullptr; }
+        PSKTM_HIP(hipMalloc(&p->slow, nv * sizeof(float)));
+        p->slow_cap = nv;
+        p->bytes += nv * sizeof(float);
+    }
+    if (p->tsq_cap < (size_t)g.nt) {
+        if (p->tsq) { (void)hipFree(p->tsq); p->bytes -= p->tsq_cap * sizeof(float); p->tsq = nullptr; }
+        PSKTM_HIP(hipMalloc(&p->tsq, (size_t)g.nt * sizeof(float)));
+        p->tsq_cap = g.nt;
+        p->bytes += (size_t)g.nt * sizeof(float);
+    }
+    const double inv_dt = 1.0 / (double)p->dt;
+    const size_t n = std::max(nv, (size_t)g.nt);
+    hipLaunchKernelGGL(tables_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g.nt,
+                       g.tau0, g.dtau, (float)inv_dt, (float)(inv_dt * inv_dt), vel.v, nv,
+                       p->slow, p->tsq);
+    return check_launch("tables_kernel");
+}
+
+static psktm_status validate_velocity(const psktm_velocity &vel)
+{
+    PSKTM_CHECK(vel.v, PSKTM_E_INVAL, "velocity pointer is NULL");
+    PSKTM_CHECK(vel.kind == PSKTM_VEL_1D || vel.kind == PSKTM_VEL_3D, PSKTM_E_INVAL,
+                "unknown velocity kind %d", vel.kind);
+    return PSKTM_OK;
+}
+
+psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_velocity &vel,
+                            float *image, int accumulate, hipStream_t st)
+{
+    psktm_status s;
+    if ((s = validate_velocity(vel)) != PSKTM_OK) return s;
+    PSKTM_CHECK(tr.dtype == PSKTM_F32 || tr.dtype == PSKTM_F16 || tr.dtype == PSKTM_BF16,
+                PSKTM_E_INVAL, "unknown trace dtype %d", tr.dtype);
+    PSKTM_CHECK(p->n_traces == 0 || tr.data, PSKTM_E_INVAL, "trace data is NULL");
+    PSKTM_CHECK(tr.dtype == PSKTM_F32 || p->n_traces == 0 || tr.scale, PSKTM_E_INVAL,
+                "16-bit traces need per-trace scales");
+    const psktm_grid &g = p->grid;
+    const int tb = p->tile_x * p->tile_y;
+    const bool v3 = vel.kind == PSKTM_VEL_3D;
+    const FwdShape sh = choose_shape(g.nt, tb, v3);
+    FwdFn fn = fwd_kernel(sh.nth, tb, sh.tl, v3);
+    PSKTM_CHECK(fn, PSKTM_E_UNSUPPORTED, "no forward kernel for tile %d tl %d", tb, sh.tl);
+
+    const int nsp = (p->n_samples + 2) & ~1;
+    const size_t slot = (size_t)nsp * sizeof(float2);
+    int G = p->opt.traces_per_stage > 0 ? p->opt.traces_per_stage : 4;
+    G = std::min(G, kGMax);
+    while (G > 1 && (size_t)G * slot > 96 * 1024) --G;
+    PSKTM_CHECK((size_t)G * slot <= 150 * 1024, PSKTM_E_UNSUPPORTED,
+                "traces of %d samples do not fit in LDS", p->n_samples);
+    if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
+
+    FwdArgs a{};
+    a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
+    a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;
+    a.tile_x = p->tile_x; a.tile_y = p->tile_y; a.tiles_y = p->tiles_y; a.n_chunks = sh.chunks;
+    a.nsf = (float)p->n_samples;
+    a.r2 = p->opt.aperture * p->opt.aperture;
+    a.wscale = (float)std::sqrt((double)p->dt);
+    a.use_ap = !p->all_traces;
+    a.all_traces = p->all_traces;
+    a.ntr = p->n_traces;
+    a.interp = p->opt.interp;
+    a.dtype = tr.dtype;
+    a.G = G;
+    a.nsp = nsp;
+    a.accumulate = accumulate;
+    a.geo = p->geo; a.list = p->list; a.tile_off = p->tile_off; a.tile_order = p->tile_order;
+    a.traces = tr.data; a.scale = tr.scale; a.slow = p->slow; a.tsq = p->tsq; a.image = image;
+    const size_t lds = (size_t)G * slot;
+    if (lds > 64 * 1024)
+        PSKTM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    const unsigned blocks = (unsigned)((size_t)p->n_tiles * sh.chunks);
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(sh.nth), lds, st, a);
+    return check_launch("migrate_fwd_kernel");
+}
+
+psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_velocity &vel,
+                              float *traces_out, int accumulate, hipStream_t st)
+{
+    psktm_status s;
+    if ((s = validate_velocity(vel)) != PSKTM_OK) return s;
+    const psktm_grid &g = p->grid;
+    const int tb = p->tile_x * p->tile_y;
+    const bool v3 = vel.kind == PSKTM_VEL_3D;
+    const int nsa = p->n_samples + 2;
+    const size_t lds = (size_t)kAdjMax * nsa * sizeof(float);
+    PSKTM_CHECK(lds <= 150 * 1024, PSKTM_E_UNSUPPORTED,
+                "traces of %d samples do not fit in LDS for the adjoint", p->n_samples);
+    if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
+    const size_t ntot = (size_t)p->n_traces * p->n_samples;
+    if (!accumulate && ntot)
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st,
+                           traces_out, ntot);
+    if (p->n_adj_items == 0) return check_launch("zero_kernel");
+
+    constexpr int NTH = 256, TL = 8;
+    AdjFn fn = (tb == 8) ? (v3 ? demigrate_kernel<NTH, 8, TL, true> : demigrate_kernel<NTH, 8, TL, false>)
+                         : (v3 ? demigrate_kernel<NTH, 16, TL, true> : demigrate_kernel<NTH, 16, TL, false>);
+    AdjArgs a{};
+    a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
+    a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;
+    a.tile_x = p->tile_x; a.tile_y = p->tile_y; a.tiles_x = p->tiles_x; a.tiles_y = p->tiles_y;
+    a.n_chunks = (g.nt + NTH * TL - 1) / (NTH * TL);
+    a.cells_y = p->cells_y; a.cell_mx = p->cell_mx; a.cell_my = p->cell_my;
+    a.win_x = p->win_x; a.win_y = p->win_y;
+    a.nsf = (float)p->n_samples;
+    a.r2 = p->opt.aperture * p->opt.aperture;
+    a.wscale = (float)std::sqrt((double)p->dt);
+    a.use_ap = !p->all_traces;
+    a.interp = p->opt.interp;
+    a.nsa = nsa;
+    a.geo = p->geo; a.sorted_ids = p->sorted_ids; a.cell_start = p->cell_start;
+    a.items = p->adj_items; a.slow = p->slow; a.tsq = p->tsq; a.image = image; a.traces = traces_out;
+    if (lds > 64 * 1024)
+        PSKTM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(fn, dim3(p->n_adj_items), dim3(NTH), lds, st, a);
+    return check_launch("demigrate_kernel");
+}
+
+} // namespace psktm
+
+using namespace psktm;
+
+extern "C" {
+
+psktm_status psktm_migrate(psktm_plan *plan, const psktm_traces *traces,
+                           const psktm_velocity *vel, float *image, int accumulate,
+                           void *stream)
+{
+    psktm__clear_error();
+    PSKTM_CHECK(plan && traces && vel && image, PSKTM_E_INVAL, "NULL argument");
+    return launch_migrate(plan, *traces, *vel, image, accumulate, as_stream(stream));
+}
+
+psktm_status psktm_demigrate(psktm_plan *plan, const float *image, const psktm_velocity *vel,
+                             float *traces_out, int accumulate, void *stream)
+{
+    psktm__clear_error();
+    PSKTM_CHECK(plan && image && vel && (traces_out || plan->n_traces == 0), PSKTM_E_INVAL,
+                "NULL argument");
+    return launch_demigrate(plan, image, *vel, traces_out, accumulate, as_stream(stream));
+}
+
+} // extern "C"

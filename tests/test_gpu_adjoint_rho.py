@@ -1,0 +1,153 @@
+"""Adjoint (demigration), rho filter, quantisation and FP64 reductions on the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import psktm
+from psktm import synth
+
+from test_gpu_migrate import gpu_plan, rel_l2, small_3d
+
+pytestmark = pytest.mark.gpu
+
+
+def dot64(a, b):
+    return float(np.sum(np.asarray(a, np.float64) * np.asarray(b, np.float64)))
+
+
+@pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
+def test_dot_product_test_3d(cuda, interp):
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=11, ntr=2000, nx=10, ny=9)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, interp=interp)
+    g = torch.Generator(device="cpu").manual_seed(1)
+    d = torch.randn(tr.shape, generator=g).to(cuda)
+    m = torch.randn(grid[0], grid[3], grid[6], generator=g).to(cuda)
+    Fd = psktm.migrate(plan, d, t(vel))
+    Ftm = psktm.demigrate(plan, m, t(vel))
+    lhs = psktm.dot_f64(Fd.reshape(-1), m.reshape(-1)).item()
+    rhs = psktm.dot_f64(d.reshape(-1), Ftm.reshape(-1)).item()
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"dot-product test interp={interp}: <Fd,m>={lhs:.9e} <d,F'm>={rhs:.9e} rel={rel:.2e}")
+    assert rel <= 1e-6
+
+
+def test_adjoint_vs_fp64_oracle(cuda):
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=12, ntr=600, nx=8, ny=7, nt=200, ns=240)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap)
+    rng = np.random.default_rng(2)
+    m = rng.standard_normal((grid[0], grid[3], grid[6])).astype(np.float32)
+    got = psktm.demigrate(plan, t(m), t(vel)).cpu().numpy()
+    ref = oracle.pstm_adj_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], m,
+                              vel, dt, tr.shape[1], vel3d=True, aperture=ap,
+                              interp=oracle.LINEAR)
+    assert rel_l2(got, ref) <= 1e-4
+
+
+def test_dot_product_line_2d_no_aperture(cuda):
+    import torch
+    d = synth.line_2d(n_cmp=16, n_off=64, ns=400, nt=400)
+    mid = np.stack([d.mid_x, d.mid_y], 1)
+    off = np.stack([d.off_x, d.off_y], 1)
+    plan, t = gpu_plan(cuda, d.grid, mid, off, d.traces.shape[1], d.dt, 0.0)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    x = torch.randn(d.traces.shape, generator=g).to(cuda)
+    m = torch.randn(d.grid[0], 1, d.grid[6], generator=g).to(cuda)
+    v = t(d.vel)
+    lhs = psktm.dot_f64(psktm.migrate(plan, x, v).reshape(-1), m.reshape(-1)).item()
+    rhs = psktm.dot_f64(x.reshape(-1), psktm.demigrate(plan, m, v).reshape(-1)).item()
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+# ----------------------------- rho filter --------------------------------
+
+@pytest.mark.parametrize("opt", [1, 2, 3])
+@pytest.mark.parametrize("n,m", [(6, 1000), (5, 1500), (1, 37)])
+def test_rho_filter_vs_oracle(cuda, opt, n, m):
+    import torch
+    rng = np.random.default_rng(opt * 100 + n)
+    x = rng.standard_normal((n, m)).astype(np.float32)
+    dw = 4000.0
+    got = psktm.rho_filter(torch.as_tensor(x).to(cuda), dw, opt).cpu().numpy()
+    ref = oracle.rho_filter_per_trace(x, dw, opt)
+    assert rel_l2(got, ref) <= 2e-6
+    shifted = psktm.rho_filter(torch.as_tensor(x).to(cuda), dw, opt, matlab_trace_shift=True).cpu().numpy()
+    assert rel_l2(shifted, oracle.apply_rho_filter(x, dw, opt)) <= 2e-6
+
+
+def test_rho_closed_form_cosine_on_gpu(cuda):
+    import torch
+    M, k0, dw = 2048, 77, 4000.0
+    n = np.arange(M)
+    th = 2 * np.pi * k0 * n / M
+    x = np.cos(th).astype(np.float32)[None].repeat(2, 0)
+    got = psktm.rho_filter(torch.as_tensor(x).to(cuda), dw, 2).cpu().numpy()
+    w0 = 2 * np.pi * k0 / (M * dw * 1e-6)
+    expect = np.sqrt(M * w0) * np.cos(th + np.pi / 4)
+    np.testing.assert_allclose(got[0], expect, atol=2e-5 * np.sqrt(M * w0))
+    np.testing.assert_allclose(got[1], expect, atol=2e-5 * np.sqrt(M * w0))
+
+
+@pytest.mark.parametrize("kind,tol", [("f16", 2 ** -11), ("bf16", 2 ** -8)])
+def test_rho_fused_quantisation(cuda, kind, tol):
+    import torch
+    dt = torch.float16 if kind == "f16" else torch.bfloat16
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal((7, 1500)) * rng.uniform(1e-3, 1e3, (7, 1))).astype(np.float32)
+    xt = torch.as_tensor(x).to(cuda)
+    f32 = psktm.rho_filter(xt, 4000.0, 2)
+    q, sc = psktm.rho_filter(xt, 4000.0, 2, out_dtype=dt)
+    deq = q.float() * sc[:, None]
+    err = (deq - f32).abs() / f32.abs().amax(1, keepdim=True)
+    assert err.max().item() <= tol
+    q2, sc2 = psktm.quantize(f32, dt)
+    assert torch.equal(sc2, sc)
+    qmax = 32768.0 if kind == "f16" else 1.0
+    assert torch.allclose(sc, f32.abs().amax(1) / qmax, rtol=1e-6)
+
+
+def test_quantize_zero_trace(cuda):
+    import torch
+    x = torch.zeros((3, 100), device=cuda)
+    x[1, 5] = -2.5
+    q, sc = psktm.quantize(x, torch.float16)
+    assert sc[0].item() == 0 and sc[2].item() == 0
+    assert (q[1].float() * sc[1]).tolist()[5] == -2.5
+    assert not torch.any(q[0]) and not torch.any(q[2])
+
+
+# ----------------------------- misc ----------------------------------------
+
+def test_ibm_device_matches_host_exact(cuda):
+    import torch
+    rng = np.random.default_rng(8)
+    w = rng.integers(0, 2 ** 32, size=100000, dtype=np.uint64).astype(np.uint32)
+    host = psktm.ibm32_to_ieee_exact(w)
+    dev = psktm.ibm32_to_ieee_device(torch.as_tensor(w.view(np.int32)).to(cuda)).cpu().numpy()
+    assert np.array_equal(host.view(np.uint32), dev.view(np.uint32))
+
+
+def test_fp64_dot_and_ratio_updates(cuda):
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(4)
+    a = torch.randn(1_000_003, generator=g)
+    b = torch.randn(1_000_003, generator=g)
+    r = psktm.dot_f64(a.to(cuda), b.to(cuda)).item()
+    assert abs(r - float((a.double() * b.double()).sum())) <= 1e-12 * a.numel()
+    y = a.clone().to(cuda)
+    num = torch.tensor([3.0], dtype=torch.float64, device=cuda)
+    den = torch.tensor([2.0], dtype=torch.float64, device=cuda)
+    L = psktm.load()
+    import ctypes
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert L.psktm_axpy_ratio(ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(b.to(cuda).data_ptr()),
+                              a.numel(), ctypes.c_void_p(num.data_ptr()),
+                              ctypes.c_void_p(den.data_ptr()), -1.0, st) == 0
+    assert torch.allclose(y.cpu(), a - 1.5 * b, rtol=1e-6, atol=1e-6)
+    p = a.clone().to(cuda)
+    assert L.psktm_xpby_ratio(ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(b.to(cuda).data_ptr()),
+                              a.numel(), ctypes.c_void_p(num.data_ptr()),
+                              ctypes.c_void_p(den.data_ptr()), st) == 0
+    assert torch.allclose(p.cpu(), b + 1.5 * a, rtol=1e-6, atol=1e-6)

@@ -1,0 +1,165 @@
+"""Fast HIP migration (plan + psktm_migrate) against the FP64 oracle.
+
+Tolerances (relative L2 over the compared image, FP64 reference):
+  LINEAR + FAST + FP32 traces   <= 1e-4
+  LINEAR + FAST + FP16 traces   <= 1e-3   (the BASELINE metric's bound)
+  LINEAR + FAST + BF16 traces   <= 8e-3   (reported form, 8-bit mantissa)
+  TRUNC  + FAST + FP32 traces   <= 2e-2   (index flips near sample
+                                           boundaries; smooth data)
+"""
+import numpy as np
+import pytest
+
+import oracle
+import psktm
+from psktm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def small_3d(seed=3, ntr=3000, nx=13, ny=11, nt=300, ns=320, aperture=400.0, dxy=25.0):
+    rng = np.random.default_rng(seed)
+    ext = 8 * dxy
+    mid = rng.uniform(-ext, (nx - 1) * dxy + ext, (ntr, 2)).astype(np.float32)
+    mid[:, 1] = rng.uniform(-ext, (ny - 1) * dxy + ext, ntr).astype(np.float32)
+    off = rng.uniform(-300, 300, (ntr, 2)).astype(np.float32)
+    n = np.arange(ns)
+    f = rng.uniform(0.01, 0.06, (ntr, 1))
+    ph = rng.uniform(0, 2 * np.pi, (ntr, 1))
+    tr = (np.sin(2 * np.pi * f * n[None] + ph) * rng.uniform(0.5, 2.0, (ntr, 1))).astype(np.float32)
+    grid = (nx, 0.0, dxy, ny, 0.0, dxy, nt, 0.0, 0.002)
+    xs = np.arange(nx, dtype=np.float32) * np.float32(dxy)
+    ys = np.arange(ny, dtype=np.float32) * np.float32(dxy)
+    t0 = 2 * np.arange(nt, dtype=np.float32) * np.float32(0.002)
+    vel = (1500 + 600 * t0[None, None, :] + 200 * np.sin(xs[:, None, None] / 200.0) *
+           np.cos(ys[None, :, None] / 300.0)).astype(np.float32)
+    return grid, mid, off, tr, vel, 0.004, aperture
+
+
+def gpu_plan(dev, grid, mid, off, ns, dt, aperture, interp=psktm.INTERP_LINEAR, **kw):
+    import torch
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)
+    s = psktm.Survey(t(mid[:, 0]), t(mid[:, 1]), t(off[:, 0]), t(off[:, 1]), ns, dt)
+    return psktm.Plan(psktm.Grid(*grid), s, interp=interp, aperture=aperture, **kw), t
+
+
+def oracle_f64(grid, mid, off, tr, vel, dt, aperture, interp, vel3d=True, bins=None):
+    return oracle.pstm_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], tr,
+                           vel, dt, vel3d=vel3d, aperture=aperture, interp=interp, bins=bins,
+                           threads=0)
+
+
+def test_linear_fp32_3d_vs_fp64(cuda):
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d()
+    print(f"case linear fp32 3d grid={grid} ntr={len(mid)}", flush=True)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap)
+    img = psktm.migrate(plan, t(tr), t(vel)).cpu().numpy()
+    ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+    e = rel_l2(img, ref)
+    print(f"rel-L2 = {e:.3e}")
+    assert e <= 1e-4
+    # exact update count: in-aperture (trace, bin) pairs x nt, binary32 rule
+    xs = np.float32(0.0) + np.arange(grid[0], dtype=np.float32) * np.float32(grid[2])
+    ys = np.float32(0.0) + np.arange(grid[3], dtype=np.float32) * np.float32(grid[5])
+    ex = mid[:, 0][:, None, None] - xs[None, :, None]
+    ey = mid[:, 1][:, None, None] - ys[None, None, :]
+    pairs = np.count_nonzero((ex * ex + ey * ey) <= np.float32(ap) * np.float32(ap))
+    assert plan.updates() == pairs * grid[6]
+    # deterministic
+    img2 = psktm.migrate(plan, t(tr), t(vel)).cpu().numpy()
+    assert np.array_equal(img, img2)
+
+
+def test_fp16_bf16_traces_vs_fp64(cuda):
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=4)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap)
+    ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+    errs = {}
+    for dtype, tol in ((torch.float16, 1e-3), (torch.bfloat16, 8e-3)):
+        q, sc = psktm.quantize(t(tr), dtype)
+        img = psktm.migrate(plan, q, t(vel), scale=sc).cpu().numpy()
+        errs[str(dtype)] = rel_l2(img, ref)
+        assert errs[str(dtype)] <= tol, errs
+    print("rel-L2", errs)
+
+
+def test_trunc_fast_vs_reference_restatement(cuda):
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=5, ntr=1500)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, interp=psktm.INTERP_TRUNC)
+    img = psktm.migrate(plan, t(tr), t(vel)).cpu().numpy()
+    ref32 = oracle.pstm_f32(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], tr,
+                            vel, dt, vel3d=True, aperture=ap)
+    ref64 = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.TRUNC)
+    e32, e64 = rel_l2(img, ref32), rel_l2(img, ref64)
+    print(f"TRUNC+FAST rel-L2 vs binary32 reference {e32:.3e}, vs FP64 {e64:.3e}")
+    assert e32 <= 2e-2 and e64 <= 2e-2
+
+
+def test_line_2d_no_aperture_vs_fp64(cuda):
+    d = synth.line_2d(n_cmp=32, n_off=16, ns=500, nt=500)
+    mid = np.stack([d.mid_x, d.mid_y], 1)
+    off = np.stack([d.off_x, d.off_y], 1)
+    plan, t = gpu_plan(cuda, d.grid, mid, off, d.traces.shape[1], d.dt, 0.0)
+    img = psktm.migrate(plan, t(d.traces), t(d.vel)).cpu().numpy()
+    ref = oracle_f64(d.grid, mid, off, d.traces, d.vel, d.dt, 0.0, oracle.LINEAR, vel3d=False)
+    assert rel_l2(img, ref) <= 1e-4
+    assert plan.updates() == len(mid) * d.grid[0] * d.grid[6]
+
+
+@pytest.mark.parametrize("tile,stage", [((4, 4), 1), ((8, 2), 3), ((2, 4), 8), ((4, 2), 4)])
+def test_tile_shapes_and_stages_agree(cuda, tile, stage):
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=6, ntr=1200, nx=9, ny=7)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, tile_x=tile[0], tile_y=tile[1],
+                       traces_per_stage=stage)
+    img = psktm.migrate(plan, t(tr), t(vel)).cpu().numpy()
+    ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+    assert rel_l2(img, ref) <= 1e-4
+
+
+def test_accumulate_and_1d_velocity(cuda):
+    import torch
+    grid, mid, off, tr, _, dt, ap = small_3d(seed=7, ntr=800, nx=6, ny=5)
+    v1 = np.linspace(1500, 3000, grid[6]).astype(np.float32)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap)
+    a = psktm.migrate(plan, t(tr), t(v1))
+    b = a.clone()
+    psktm.migrate(plan, t(tr), t(v1), image=b, accumulate=True)
+    assert torch.allclose(b, 2 * a, rtol=1e-6, atol=0)
+    ref = oracle_f64(grid, mid, off, tr, v1, dt, ap, oracle.LINEAR, vel3d=False)
+    assert rel_l2(a.cpu().numpy(), ref) <= 1e-4
+
+
+def test_traces_outside_aperture_and_empty(cuda):
+    import torch
+    grid = (5, 0.0, 20.0, 4, 0.0, 20.0, 64, 0.0, 0.004)
+    mid = np.array([[5000.0, 5000.0], [-3000.0, 10.0]], np.float32)
+    off = np.zeros((2, 2), np.float32)
+    tr = np.ones((2, 80), np.float32)
+    plan, t = gpu_plan(cuda, grid, mid, off, 80, 0.004, 200.0)
+    img = psktm.migrate(plan, t(tr), torch.full((64,), 2000.0, device=cuda))
+    assert plan.updates() == 0 and not torch.any(img)
+    e = np.zeros((0, 2), np.float32)
+    plan0, t = gpu_plan(cuda, grid, e, e, 80, 0.004, 200.0)
+    img0 = psktm.migrate(plan0, torch.zeros((0, 80), device=cuda), torch.full((64,), 2000.0, device=cuda))
+    assert not torch.any(img0)
+
+
+def test_diffractor_survey_small(cuda):
+    """The 3-D generator at reduced size: migrated energy focuses, FP64 parity."""
+    s = synth.diffractor_3d(4000, nx=24, ny=20, nt=400, ns=420, n_diff=6, aperture=300.0,
+                            device="cpu")
+    mid = np.stack([s.mid_x.numpy(), s.mid_y.numpy()], 1)
+    off = np.stack([s.off_x.numpy(), s.off_y.numpy()], 1)
+    tr = s.traces.numpy()
+    vel = s.vel.numpy()
+    plan, t = gpu_plan(cuda, s.grid, mid, off, tr.shape[1], s.dt, s.aperture)
+    img = psktm.migrate(plan, t(tr), t(vel)).cpu().numpy()
+    ref = oracle_f64(s.grid, mid, off, tr, vel, s.dt, s.aperture, oracle.LINEAR,
+                     bins=(8, 16, 6, 14))
+    assert rel_l2(img[8:16, 6:14], ref) <= 1e-4
