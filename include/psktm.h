@@ -184,6 +184,10 @@ typedef struct psktm_options {
     float aperture;           /* metres, midpoint-to-bin; <= 0 means none */
     int   tile_x, tile_y;     /* bins per tile; 0 = library choice */
     int   traces_per_stage;   /* traces staged in LDS at once; 0 = choice */
+    int   precise;            /* 1: compensated (Kahan) FP32 accumulation in
+                                 the forward, FP64 accumulation in the
+                                 adjoint; for least squares and adjoint
+                                 tests. 0: plain FP32 accumulation. */
 } psktm_options;
 
 /* Last error text of the calling thread ("" if none). */

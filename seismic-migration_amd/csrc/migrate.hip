@@ -115,7 +115,7 @@ __device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, fl
     return m;
 }
 
-template <int NTH, int TB, int TL, bool V3>
+template <int NTH, int TB, int TL, bool V3, bool KH>
 __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 {
     extern __shared__ float2 s_pairs[];
@@ -132,6 +132,7 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
     float tq[TL];
     float sv[V3 ? TB : 1][TL];
     float acc[TB][TL];
+    float cmp[KH ? TB : 1][KH ? TL : 1];   // Kahan compensation (precise mode)
 #pragma unroll
     for (int j = 0; j < TL; ++j) {
         const int l = lbase + j * NTH;
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 #pragma unroll
         for (int j = 0; j < TL; ++j) {
             acc[b][j] = 0.f;
+            if (KH) cmp[KH ? b : 0][KH ? j : 0] = 0.f;
             if (V3) {
                 const int l = lbase + j * NTH;
                 sv[V3 ? b : 0][j] = (bin_ok && l < a.nt)
@@ -208,7 +210,15 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
                     const float f = __builtin_amdgcn_fractf(x);
                     const float2 pr = P[n];
                     const float d = fmaf(f, pr.y, pr.x);
-                    acc[b][j] = fmaf(d, __builtin_amdgcn_sqrtf(x), acc[b][j]);
+                    if (KH) {
+                        float &c = cmp[KH ? b : 0][KH ? j : 0];
+                        const float y = fmaf(d, __builtin_amdgcn_sqrtf(x), -c);
+                        const float t = acc[b][j] + y;
+                        c = (t - acc[b][j]) - y;
+                        acc[b][j] = t;
+                    } else {
+                        acc[b][j] = fmaf(d, __builtin_amdgcn_sqrtf(x), acc[b][j]);
+                    }
                 }
             }
         }
@@ -226,16 +236,18 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
             if (l >= a.nt) continue;
             const float w = axis_f32(a.tau0, a.dtau, l) * a.wscale;
             const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
-            const float v = acc[b][j] * w;
+            const float sum = KH ? acc[b][j] - cmp[KH ? b : 0][KH ? j : 0] : acc[b][j];
+            const float v = sum * w;
             a.image[idx] = a.accumulate ? a.image[idx] + v : v;
         }
     }
 }
 
-template <int NTH, int TB, int TL, bool V3>
+template <int NTH, int TB, int TL, bool V3, typename Acc>
 __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
 {
-    extern __shared__ float s_acc[];
+    extern __shared__ __align__(8) unsigned char s_raw[];
+    Acc *s_acc = reinterpret_cast<Acc *>(s_raw);
     __shared__ float2 s_ab[kAdjMax][TB];
     __shared__ unsigned s_mask[kAdjMax];
     __shared__ int s_k[kAdjMax];
@@ -244,7 +256,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const int c = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
     const int p1 = min(p0 + kAdjMax, a.cell_start[c + 1]);
     const int ng = p1 - p0;
-    for (int q = tid; q < kAdjMax * a.nsa; q += NTH) s_acc[q] = 0.f;
+    for (int q = tid; q < kAdjMax * a.nsa; q += NTH) s_acc[q] = Acc(0);
     if (tid < kAdjMax) s_k[tid] = (tid < ng) ? a.sorted_ids[p0 + tid] : -1;
     __syncthreads();
 
@@ -303,7 +315,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                             const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
                             if (!((m >> b) & 1u)) continue;
                             const float2 ab = s_ab[g][b];
-                            float *acc = s_acc + g * a.nsa;
+                            Acc *acc = s_acc + g * a.nsa;
 #pragma unroll
                             for (int j = 0; j < TL; ++j) {
                                 const float ua = fmaf(ab.x, sl[j], tq[j]);
@@ -316,10 +328,10 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                 n = (n >= lo && n <= hi) ? n : a.ns;   // a.ns, a.ns+1: discarded slots
                                 if (linear) {
                                     const float w1 = w * f;
-                                    atomicAdd(acc + n, w - w1);
-                                    atomicAdd(acc + n + 1, w1);
+                                    atomicAdd(acc + n, Acc(w - w1));
+                                    atomicAdd(acc + n + 1, Acc(w1));
                                 } else {
-                                    atomicAdd(acc + n, w);
+                                    atomicAdd(acc + n, Acc(w));
                                 }
                             }
                         }
@@ -331,8 +343,8 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     }
     for (int g = 0; g < ng; ++g) {
         float *out = a.traces + (size_t)s_k[g] * a.ns;
-        const float *acc = s_acc + g * a.nsa;
-        for (int n = tid; n < a.ns; n += NTH) out[n] += acc[n];
+        const Acc *acc = s_acc + g * a.nsa;
+        for (int n = tid; n < a.ns; n += NTH) out[n] = (float)((Acc)out[n] + acc[n]);
     }
 }
 
@@ -362,48 +374,66 @@ __global__ void zero_kernel(float *p, size_t n)
 using FwdFn = void (*)(FwdArgs);
 using AdjFn = void (*)(AdjArgs);
 
-template <int NTH, int TB, bool V3>
+// Per-lane register budget for the accumulators, the Kahan compensations and
+// the 3-D slowness samples; shapes beyond it are not instantiated.
+constexpr int acc_regs(int tb, int tl, bool v3, bool kh)
+{
+    return tb * tl * ((kh ? 2 : 1) + (v3 ? 1 : 0));
+}
+constexpr int reg_budget(bool kh) { return kh ? 128 : 96; }
+
+template <int NTH, int TB, int TL, bool V3, bool KH>
+FwdFn fwd_pick()
+{
+    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH))
+        return migrate_fwd_kernel<NTH, TB, TL, V3, KH>;
+    else
+        return nullptr;
+}
+
+template <int NTH, int TB, bool V3, bool KH>
 FwdFn fwd_by_tl(int tl)
 {
     switch (tl) {
-    case 2: return migrate_fwd_kernel<NTH, TB, 2, V3>;
-    case 3: return migrate_fwd_kernel<NTH, TB, 3, V3>;
-    case 4: return migrate_fwd_kernel<NTH, TB, 4, V3>;
-    case 6: return migrate_fwd_kernel<NTH, TB, 6, V3>;
-    case 8: return migrate_fwd_kernel<NTH, TB, 8, V3>;
+    case 2: return fwd_pick<NTH, TB, 2, V3, KH>();
+    case 3: return fwd_pick<NTH, TB, 3, V3, KH>();
+    case 4: return fwd_pick<NTH, TB, 4, V3, KH>();
+    case 6: return fwd_pick<NTH, TB, 6, V3, KH>();
+    case 8: return fwd_pick<NTH, TB, 8, V3, KH>();
     default: return nullptr;
     }
 }
 
-FwdFn fwd_kernel(int nth, int tb, int tl, bool v3)
+template <int NTH, int TB>
+FwdFn fwd_by_flags(int tl, bool v3, bool kh)
 {
-    if (nth == 256) {
-        if (tb == 8) return v3 ? fwd_by_tl<256, 8, true>(tl) : fwd_by_tl<256, 8, false>(tl);
-        return v3 ? fwd_by_tl<256, 16, true>(tl) : fwd_by_tl<256, 16, false>(tl);
-    }
-    if (tb == 8) return v3 ? fwd_by_tl<512, 8, true>(tl) : fwd_by_tl<512, 8, false>(tl);
-    return v3 ? fwd_by_tl<512, 16, true>(tl) : fwd_by_tl<512, 16, false>(tl);
+    if (v3) return kh ? fwd_by_tl<NTH, TB, true, true>(tl) : fwd_by_tl<NTH, TB, true, false>(tl);
+    return kh ? fwd_by_tl<NTH, TB, false, true>(tl) : fwd_by_tl<NTH, TB, false, false>(tl);
 }
 
-// Register budget per lane for accumulators (+ 3-D slowness): TB*TL*(1+V3).
-constexpr int kRegBudget = 96;
+FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh)
+{
+    if (nth == 256) return tb == 8 ? fwd_by_flags<256, 8>(tl, v3, kh) : fwd_by_flags<256, 16>(tl, v3, kh);
+    return tb == 8 ? fwd_by_flags<512, 8>(tl, v3, kh) : fwd_by_flags<512, 16>(tl, v3, kh);
+}
+
 constexpr int kTLSet[] = {2, 3, 4, 6, 8};
 
 struct FwdShape {
     int nth, tl, chunks;
 };
 
-FwdShape choose_shape(int nt, int tb, bool v3)
+FwdShape choose_shape(int nt, int tb, bool v3, bool kh)
 {
-    const int tlmax = std::min(8, kRegBudget / (tb * (v3 ? 2 : 1)));
+    auto fits = [&](int tl) { return acc_regs(tb, tl, v3, kh) <= reg_budget(kh); };
     for (int nth : {256, 512}) {
         const int need = (nt + nth - 1) / nth;
         for (int tl : kTLSet)
-            if (tl >= need && tl <= tlmax) return {nth, tl, 1};
+            if (tl >= need && fits(tl)) return {nth, tl, 1};
     }
     int tl = 2;
     for (int t : kTLSet)
-        if (t <= tlmax) tl = t;
+        if (fits(t)) tl = t;
     return {512, tl, (nt + 512 * tl - 1) / (512 * tl)};
 }
 
@@ -454,8 +484,9 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     const psktm_grid &g = p->grid;
     const int tb = p->tile_x * p->tile_y;
     const bool v3 = vel.kind == PSKTM_VEL_3D;
-    const FwdShape sh = choose_shape(g.nt, tb, v3);
-    FwdFn fn = fwd_kernel(sh.nth, tb, sh.tl, v3);
+    const bool kh = p->opt.precise != 0;
+    const FwdShape sh = choose_shape(g.nt, tb, v3, kh);
+    FwdFn fn = fwd_kernel(sh.nth, tb, sh.tl, v3, kh);
     PSKTM_CHECK(fn, PSKTM_E_UNSUPPORTED, "no forward kernel for tile %d tl %d", tb, sh.tl);
 
     const int nsp = (p->n_samples + 2) & ~1;
@@ -502,7 +533,8 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     const int tb = p->tile_x * p->tile_y;
     const bool v3 = vel.kind == PSKTM_VEL_3D;
     const int nsa = p->n_samples + 2;
-    const size_t lds = (size_t)kAdjMax * nsa * sizeof(float);
+    const bool f64 = p->opt.precise != 0;
+    const size_t lds = (size_t)kAdjMax * nsa * (f64 ? sizeof(double) : sizeof(float));
     PSKTM_CHECK(lds <= 150 * 1024, PSKTM_E_UNSUPPORTED,
                 "traces of %d samples do not fit in LDS for the adjoint", p->n_samples);
     if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
@@ -513,8 +545,13 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     if (p->n_adj_items == 0) return check_launch("zero_kernel");
 
     constexpr int NTH = 256, TL = 8;
-    AdjFn fn = (tb == 8) ? (v3 ? demigrate_kernel<NTH, 8, TL, true> : demigrate_kernel<NTH, 8, TL, false>)
-                         : (v3 ? demigrate_kernel<NTH, 16, TL, true> : demigrate_kernel<NTH, 16, TL, false>);
+    AdjFn fn;
+    if (f64)
+        fn = (tb == 8) ? (v3 ? demigrate_kernel<NTH, 8, TL, true, double> : demigrate_kernel<NTH, 8, TL, false, double>)
+                       : (v3 ? demigrate_kernel<NTH, 16, TL, true, double> : demigrate_kernel<NTH, 16, TL, false, double>);
+    else
+        fn = (tb == 8) ? (v3 ? demigrate_kernel<NTH, 8, TL, true, float> : demigrate_kernel<NTH, 8, TL, false, float>)
+                       : (v3 ? demigrate_kernel<NTH, 16, TL, true, float> : demigrate_kernel<NTH, 16, TL, false, float>);
     AdjArgs a{};
     a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
     a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;

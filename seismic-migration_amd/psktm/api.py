@@ -125,11 +125,12 @@ class Plan:
     """psktm_plan: traces binned into output tiles with per-tile aperture lists."""
 
     def __init__(self, grid: Grid, survey: Survey, *, interp=INTERP_LINEAR, math=MATH_FAST,
-                 aperture=0.0, tile_x=0, tile_y=0, traces_per_stage=0, stream=None):
+                 aperture=0.0, tile_x=0, tile_y=0, traces_per_stage=0, precise=False,
+                 stream=None):
         L = _lib.load()
         self.grid, self.survey = grid, survey
         self.opt = PsktmOptions(int(interp), int(math), float(aperture), int(tile_x),
-                                int(tile_y), int(traces_per_stage))
+                                int(tile_y), int(traces_per_stage), int(bool(precise)))
         self._h = _p()
         g, s = grid.c(), survey.c()
         check(L.psktm_plan_create(ctypes.byref(self._h), ctypes.byref(g), ctypes.byref(s),

@@ -17,11 +17,18 @@ def dot64(a, b):
     return float(np.sum(np.asarray(a, np.float64) * np.asarray(b, np.float64)))
 
 
+@pytest.mark.parametrize("precise,tol", [(True, 1e-6), (False, 2e-5)])
 @pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
-def test_dot_product_test_3d(cuda, interp):
+def test_dot_product_test_3d(cuda, interp, precise, tol):
+    """<F d, m> = <d, F' m> with zero-mean random d, m and FP64 inner products.
+
+    Precise mode (Kahan forward, FP64 adjoint accumulation) meets 1e-6. Plain
+    FP32 accumulation of K ~ 2000 random-sign terms per image sample carries a
+    relative error ~ eps*sqrt(K/2) ~ 2e-6 per sample, so its bound is 2e-5.
+    """
     import torch
     grid, mid, off, tr, vel, dt, ap = small_3d(seed=11, ntr=2000, nx=10, ny=9)
-    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, interp=interp)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, interp=interp, precise=precise)
     g = torch.Generator(device="cpu").manual_seed(1)
     d = torch.randn(tr.shape, generator=g).to(cuda)
     m = torch.randn(grid[0], grid[3], grid[6], generator=g).to(cuda)
@@ -30,8 +37,9 @@ def test_dot_product_test_3d(cuda, interp):
     lhs = psktm.dot_f64(Fd.reshape(-1), m.reshape(-1)).item()
     rhs = psktm.dot_f64(d.reshape(-1), Ftm.reshape(-1)).item()
     rel = abs(lhs - rhs) / abs(lhs)
-    print(f"dot-product test interp={interp}: <Fd,m>={lhs:.9e} <d,F'm>={rhs:.9e} rel={rel:.2e}")
-    assert rel <= 1e-6
+    print(f"dot-product test interp={interp} precise={precise}: <Fd,m>={lhs:.9e} "
+          f"<d,F'm>={rhs:.9e} rel={rel:.2e}")
+    assert rel <= tol
 
 
 def test_adjoint_vs_fp64_oracle(cuda):
@@ -51,7 +59,7 @@ def test_dot_product_line_2d_no_aperture(cuda):
     d = synth.line_2d(n_cmp=16, n_off=64, ns=400, nt=400)
     mid = np.stack([d.mid_x, d.mid_y], 1)
     off = np.stack([d.off_x, d.off_y], 1)
-    plan, t = gpu_plan(cuda, d.grid, mid, off, d.traces.shape[1], d.dt, 0.0)
+    plan, t = gpu_plan(cuda, d.grid, mid, off, d.traces.shape[1], d.dt, 0.0, precise=True)
     g = torch.Generator(device="cpu").manual_seed(3)
     x = torch.randn(d.traces.shape, generator=g).to(cuda)
     m = torch.randn(d.grid[0], 1, d.grid[6], generator=g).to(cuda)
