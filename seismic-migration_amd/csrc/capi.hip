@@ -87,6 +87,26 @@ psktm_status check_launch(const char *kernel)
     return PSKTM_OK;
 }
 
+psktm_status ensure_dynamic_lds(const void *kernel, size_t bytes)
+{
+    if (bytes <= 64 * 1024) return PSKTM_OK;
+    static std::mutex m;
+    static std::vector<std::pair<std::pair<const void *, int>, size_t>> done;
+    int dev = 0;
+    PSKTM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(m);
+    for (auto &d : done)
+        if (d.first.first == kernel && d.first.second == dev && d.second >= bytes) return PSKTM_OK;
+    PSKTM_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    for (auto &d : done)
+        if (d.first.first == kernel && d.first.second == dev) {
+            d.second = bytes;
+            return PSKTM_OK;
+        }
+    done.push_back({{kernel, dev}, bytes});
+    return PSKTM_OK;
+}
+
 psktm_status validate_grid(const psktm_grid *g)
 {
     PSKTM_CHECK(g, PSKTM_E_INVAL, "grid is NULL");

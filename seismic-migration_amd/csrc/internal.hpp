@@ -40,6 +40,11 @@ psktm_status require_device();
 // Launch-error check after a kernel launch.
 psktm_status check_launch(const char *kernel);
 
+// Raises a kernel's dynamic-LDS limit to at least `bytes` (above 64 KiB).
+// The attribute is set once per kernel and size, before any graph capture
+// that replays the launch.
+psktm_status ensure_dynamic_lds(const void *kernel, size_t bytes);
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Validates a grid and survey (sizes positive, pointers present).

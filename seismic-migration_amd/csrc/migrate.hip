@@ -516,9 +516,7 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     a.geo = p->geo; a.list = p->list; a.tile_off = p->tile_off; a.tile_order = p->tile_order;
     a.traces = tr.data; a.scale = tr.scale; a.slow = p->slow; a.tsq = p->tsq; a.image = image;
     const size_t lds = (size_t)G * slot;
-    if (lds > 64 * 1024)
-        PSKTM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
+    { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }
     const unsigned blocks = (unsigned)((size_t)p->n_tiles * sh.chunks);
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(sh.nth), lds, st, a);
     return check_launch("migrate_fwd_kernel");
@@ -567,9 +565,7 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     a.nsa = nsa;
     a.geo = p->geo; a.sorted_ids = p->sorted_ids; a.cell_start = p->cell_start;
     a.items = p->adj_items; a.slow = p->slow; a.tsq = p->tsq; a.image = image; a.traces = traces_out;
-    if (lds > 64 * 1024)
-        PSKTM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
+    { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }
     hipLaunchKernelGGL(fn, dim3(p->n_adj_items), dim3(NTH), lds, st, a);
     return check_launch("demigrate_kernel");
 }

@@ -256,9 +256,7 @@ psktm_status psktm_rho_filter(const float *in, int n_traces, int n_samples, floa
     const double Dr = (double)dw_us * 1e-6;
     const float wscale = (float)(2.0 * M_PI / Dr);
     const size_t lds = sizeof(float2) * (2 * (size_t)M + M / 2);
-    if (lds > 64 * 1024)
-        PSKTM_HIP(hipFuncSetAttribute((const void *)rho_kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)rho_kernel, lds)) != PSKTM_OK) return s_; }
     const unsigned blocks = (unsigned)((n_traces + 1) / 2);
     hipLaunchKernelGGL(rho_kernel, dim3(blocks), dim3(kRhoThreads), lds, as_stream(stream), in,
                        n_traces, n_samples, M, logM, wscale, opt, out_dtype,
