@@ -66,6 +66,12 @@ def parse():
     ap.add_argument("--oracle-bins", type=int, default=4,
                     help="side of the central sub-volume compared with FP64")
     ap.add_argument("--forms", default="f16,f32", help="timed forms (f16, f32, bf16)")
+    ap.add_argument("--workload", default="pstm3d", choices=["pstm3d", "batched", "lsm", "line2d"],
+                    help="pstm3d: the headline 3-D workload (default); batched: 64 offset-class "
+                         "gathers in one persistent launch; lsm: least-squares 2-D line, 20 CG "
+                         "iterations; line2d: the 2-D line, reference-exact GPU form vs the "
+                         "reference CPU loop")
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache (batched workload)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -87,6 +93,12 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload != "pstm3d":
+        if args.gpus != 1:
+            print(json.dumps({"error": "one GPU only"}))
+            sys.exit(2)
+        {"batched": bench_batched, "lsm": bench_lsm, "line2d": bench_line2d}[args.workload](args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -229,6 +241,16 @@ def main():
                          f"{el:.1f} s)"}
         log(f"CPU baseline {cpu['value']:.3f} G updates/s on {cores} threads ({el:.1f} s)")
 
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # runs of this same workload (scaled by the copy-kernel calibration)
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "migrate_fwd_3d_1M_traces_traffic.json")
+    if os.path.exists(tp) and "f16" in results:
+        prof = json.load(open(tp))
+        if prof["workload"]["traces"] == args.traces and \
+                abs(prof["updates_per_launch"] - updates) <= 1e-9 * updates:
+            traffic = prof["hbm_bytes_per_launch"]
+
     head = results.get("f16") or next(iter(results.values()))
     f32 = results.get("f32")
     kern_s = head["kernel_ms"] / 1e3
@@ -264,7 +286,9 @@ def main():
             "peak": VALU_PEAK_LANE_OPS / 1e12,
             "unit": "T VALU lane-op/s",
             "frac": valu_ops / VALU_PEAK_LANE_OPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+            "hbm_GBps": (traffic / kern_s / 1e9) if traffic else None,
             "kernel": "migrate_fwd_kernel",
             "per_update": f"{VALU_OPS_PER_UPDATE} VALU instructions (3 v_sqrt_f32)",
             "fp32_published": {"achieved_tflops": updates * FLOPS_PER_UPDATE / kern_s / 1e12,
@@ -272,6 +296,265 @@ def main():
                                "frac": updates * FLOPS_PER_UPDATE / kern_s / 1e12 / FP32_PUBLISHED_TFLOPS},
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Secondary workloads (their own JSON line; the default run is pstm3d).
+# ---------------------------------------------------------------------------
+
+def bench_batched(args):
+    """64 offset-class gathers (lognormal 2,000-60,000 traces, patches of
+    32-128 bins at 25 m x 1,000 tau, 1,000 samples at 4 ms, aperture 1500 m),
+    FP16 traces + FP32 scales, migrated by ONE persistent launch replayed from
+    a HIP graph, tile shape from the autotuner; compared with launching the
+    same gathers one plan at a time."""
+    import numpy as np
+    import torch
+
+    import psktm
+    from psktm import synth
+    from psktm.tune import CANDIDATES_3D, Autotuner, signature, time_gpu
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    gd = synth.offset_class_gathers(64, device="cuda")
+    torch.cuda.synchronize()
+    log(f"64 gathers, {sum(gd.counts)} traces x {gd.traces.shape[1]} in {time.time() - t0:.1f} s")
+    ns = gd.traces.shape[1]
+    q, sc = psktm.rho_filter(gd.traces, gd.dt * 1e6, 2, out_dtype=torch.float16)
+    survey = psktm.Survey(gd.mid_x, gd.mid_y, gd.off_x, gd.off_y, ns, gd.dt)
+    gathers = [(psktm.Grid(*g), f, c) for g, f, c in zip(gd.grids, gd.first, gd.counts)]
+
+    # autotune on every fourth gather (a quarter of the work)
+    sub = gathers[::4]
+    key = signature("batch", ns=ns, nt=gd.grids[0][6], dtype="f16", aperture=gd.aperture,
+                    gathers=len(gathers), seed=gd.seed)
+    tuner = Autotuner(args.tune_cache)
+
+    def trial(c):
+        b = psktm.Batch(sub, survey, aperture=gd.aperture, tile_x=c[0], tile_y=c[1],
+                        traces_per_stage=c[2])
+        t = time_gpu(lambda: b.migrate(q, gd.vel, scale=sc), repeats=1)
+        log(f"  tune {c}: {t * 1e3:.1f} ms")
+        b.close()
+        return t
+
+    t1 = time.time()
+    choice = tuner.choose(key, CANDIDATES_3D, trial)
+    log(f"autotuner choice {choice} ({time.time() - t1:.1f} s, cache {tuner.cache_path})")
+    batch = psktm.Batch(gathers, survey, aperture=gd.aperture, tile_x=choice[0],
+                        tile_y=choice[1], traces_per_stage=choice[2])
+    updates = batch.updates()
+    images = torch.empty(int(sum(batch.sizes)), dtype=torch.float32, device=dev)
+    for _ in range(max(1, args.warmup)):
+        batch.migrate(q, gd.vel, images, scale=sc, use_graph=True)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.migrate(q, gd.vel, images, scale=sc, use_graph=True)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t2) / args.steps
+    # the same gathers launched one plan at a time
+    plans = []
+    for g, f, c in gathers:
+        sl = slice(f, f + c)
+        s = psktm.Survey(gd.mid_x[sl].contiguous(), gd.mid_y[sl].contiguous(),
+                         gd.off_x[sl].contiguous(), gd.off_y[sl].contiguous(), ns, gd.dt)
+        plans.append((psktm.Plan(g, s, aperture=gd.aperture, tile_x=choice[0], tile_y=choice[1],
+                                 traces_per_stage=choice[2]), sl))
+    seq_imgs = [torch.empty(g.shape, dtype=torch.float32, device=dev) for g, _, _ in gathers]
+
+    def sequential():
+        for (p, sl), img in zip(plans, seq_imgs):
+            psktm.migrate(p, q[sl], gd.vel, img, scale=sc[sl])
+
+    sequential()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        sequential()
+    torch.cuda.synchronize()
+    seq = (time.perf_counter() - t3) / args.steps
+    same = all(torch.equal(batch.image(images, k), seq_imgs[k]) for k in range(len(gathers)))
+    line = {
+        "metric": "G trace-sample updates/s (batched offset-class gathers, one persistent launch)",
+        "value": updates / wall / 1e9, "unit": "G trace-sample updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 offset-class gathers, lognormal(ln 11000, 0.8) traces clipped "
+                               "to 2,000-60,000, patches 32-128 bins at 25 m x 1,000 tau, "
+                               "1,000 samples at 4 ms, aperture 1500 m, FP16 traces",
+                   "traces": int(sum(gd.counts)), "updates_per_step": updates,
+                   "tile": list(choice), "graph": True},
+        "sequential_ms": seq * 1e3, "persistent_over_sequential": seq / wall,
+        "images_equal_sequential": bool(same),
+        "autotuner": tuner.cache.get(key),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _lsm_line(dev, n_cmp=512, n_off=64, ns=2000, dt=0.002, nt=2000, seed=None):
+    import numpy as np
+    import torch
+
+    import psktm
+    from psktm import synth
+    seed = synth.SEED0 + 4 if seed is None else seed
+    d = synth.line_2d(seed=seed, n_cmp=n_cmp, n_off=n_off, ns=ns, nt=nt, dt=dt)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)
+    tau = np.arange(nt, dtype=np.float32) * np.float32(d.grid[8])
+    vel = t(1500.0 + 500.0 * 2.0 * tau.astype(np.float64))
+    survey = psktm.Survey(t(d.mid_x), t(d.mid_y), t(d.off_x), t(d.off_y), ns, dt)
+    grid = psktm.Grid(*d.grid)
+    # reflectivity: 8 random-dip interfaces + 50 point scatterers
+    rng = np.random.default_rng(seed)
+    m = np.zeros(grid.shape, np.float32)
+    for _ in range(8):
+        l0, slope, a = rng.integers(100, nt - 100), rng.uniform(-1.0, 1.0), rng.choice([-1, 1])
+        for i in range(grid.nx):
+            l = int(round(l0 + slope * (i - grid.nx / 2)))
+            if 0 < l < nt:
+                m[i, 0, l] += a
+    for _ in range(50):
+        m[rng.integers(0, grid.nx), 0, rng.integers(50, nt - 50)] += rng.choice([-2.0, 2.0])
+    return grid, survey, vel, t(m)
+
+
+def bench_lsm(args):
+    """Least-squares 2-D PSTM: 512 CMPs x 64 offsets (32,768 traces x 2,000
+    samples at 2 ms), image 512 x 2,000 (tau step 1 ms), data = demigration of
+    8 random-dip interfaces + 50 point scatterers, 20 CGLS iterations with FP64
+    inner products (precise accumulation). Dot-product test and finite
+    differences on a random 16-CMP slice."""
+    import numpy as np
+    import torch
+
+    import psktm
+    from psktm import lsm
+
+    dev = torch.device("cuda:0")
+    grid, survey, vel, m_true = _lsm_line(dev)
+    plan = psktm.Plan(grid, survey, precise=True)
+    upd = plan.updates()
+    data = psktm.demigrate(plan, m_true, vel)
+    torch.cuda.synchronize()
+    # timing of the operators
+    tf = time.perf_counter(); psktm.migrate(plan, data, vel); torch.cuda.synchronize()
+    tf = time.perf_counter() - tf
+    ta = time.perf_counter(); psktm.demigrate(plan, m_true, vel); torch.cuda.synchronize()
+    ta = time.perf_counter() - ta
+    t0 = time.perf_counter()
+    m, hist = lsm.cgls(plan, vel, data, iters=20)
+    cg_s = time.perf_counter() - t0
+    mig = psktm.migrate(plan, data, vel)
+    cos = lambda a, b: float((a * b).sum() / (a.norm() * b.norm()))
+    # dot-product test and finite differences on a random 16-CMP slice
+    rng = np.random.default_rng(5)
+    c0 = int(rng.integers(0, 512 - 16))
+    sg, ss, sv, _ = _lsm_line(dev, n_cmp=16)
+    # shift the slice to CMPs c0..c0+15 (same offsets, same time axes)
+    shift = np.float32(12.5 * c0)
+    ss = psktm.Survey(ss.mid_x + shift, ss.mid_y, ss.off_x, ss.off_y, ss.n_samples, ss.dt)
+    sg = psktm.Grid(sg.nx, float(shift), sg.dx, sg.ny, sg.y0, sg.dy, sg.nt, sg.tau0, sg.dtau)
+    sp = psktm.Plan(sg, ss, precise=True)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    x = torch.randn((ss.n_traces, ss.n_samples), generator=g).to(dev)
+    y = torch.randn(sg.shape, generator=g).to(dev)
+    lhs = psktm.dot_f64(psktm.migrate(sp, x, sv).reshape(-1), y.reshape(-1)).item()
+    rhs = psktm.dot_f64(x.reshape(-1), psktm.demigrate(sp, y, sv).reshape(-1)).item()
+    mm = y.clone().requires_grad_(True)
+    dd = torch.randn((ss.n_traces, ss.n_samples), generator=g).to(dev)
+    e = torch.randn(sg.shape, generator=g).to(dev)
+    f = lambda v: 0.5 * ((lsm.Demigrate.apply(v, sp, sv) - dd).double() ** 2).sum()
+    L = f(mm); L.backward()
+    gd = float((mm.grad.double() * e.double()).sum())
+    with torch.no_grad():
+        fd = float((f(mm + 0.5 * e) - f(mm - 0.5 * e)) / 1.0)
+    line = {
+        "metric": "least-squares PSTM: 20 CGLS iterations (FP64 inner products)",
+        "value": 20.0 / cg_s, "unit": "CG iterations/s", "n_gpus": 1, "steps": 20, "warmup": 0,
+        "ms_per_step": cg_s / 20 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2-D line 512 CMPs x 64 offsets 0-3150 m, 32,768 traces x 2,000 "
+                               "samples (2 ms), image 512 x 2,000 (1 ms), v = 1500 + 500 t0",
+                   "updates_per_operator": upd},
+        "migrate_ms": tf * 1e3, "demigrate_ms": ta * 1e3,
+        "migrate_G_updates_per_s": upd / tf / 1e9, "demigrate_G_updates_per_s": upd / ta / 1e9,
+        "residual_norm2": hist, "residual_monotone": all(hist[k + 1] <= hist[k] * (1 + 1e-9)
+                                                         for k in range(len(hist) - 1)),
+        "cos_image_vs_model": {"cgls": cos(m, m_true), "migration": cos(mig, m_true)},
+        "dot_product_test": {"slice_first_cmp": c0, "lhs": lhs, "rhs": rhs,
+                             "rel": abs(lhs - rhs) / abs(lhs)},
+        "finite_difference": {"fd": fd, "grad_dot": gd, "rel": abs(fd - gd) / abs(gd)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_line2d(args):
+    """The 2-D line (64 CMPs x 32 offsets, 1,000 samples at 4 ms): the
+    reference-exact GPU form (bit-identical to the reference loop) timed
+    against the reference's CPU loop on the host (full workload, best of 3),
+    plus the fast LINEAR form."""
+    import numpy as np
+    import torch
+
+    import oracle
+    import psktm
+    from psktm import synth
+
+    dev = torch.device("cuda:0")
+    d = synth.line_2d()
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)
+    grid = psktm.Grid(*d.grid)
+    ns = d.traces.shape[1]
+    survey = psktm.Survey(t(d.mid_x), t(d.mid_y), t(d.off_x), t(d.off_y), ns, d.dt)
+    tr, vel = t(d.traces), t(d.vel)
+    updates = float(d.traces.shape[0]) * grid.nx * grid.ny * grid.nt
+    img = psktm.migrate_exact(grid, survey, tr, vel)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(max(3, args.steps)):
+        t0 = time.perf_counter()
+        psktm.migrate_exact(grid, survey, tr, vel, img)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    exact_s = min(ts)
+    plan = psktm.Plan(grid, survey)
+    psktm.migrate(plan, tr, vel)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(3, args.steps)):
+        fast = psktm.migrate(plan, tr, vel)
+    torch.cuda.synchronize()
+    fast_s = (time.perf_counter() - t0) / max(3, args.steps)
+    cores = len(os.sched_getaffinity(0))
+    cpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ref = oracle.pstm_f32(oracle.Grid(*d.grid), d.mid_x, d.mid_y, d.off_x, d.off_y, d.traces,
+                              d.vel, d.dt, threads=cores)
+        cpu.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    oracle.ktm_migration_racy(oracle.Grid(*d.grid), d.mid_x, d.mid_y, d.off_x, d.off_y,
+                              d.traces, d.vel, d.dt, threads=cores)
+    racy = time.perf_counter() - t0
+    bit = bool(np.array_equal(img.cpu().numpy().view(np.uint32), ref.view(np.uint32)))
+    line = {
+        "metric": "G trace-sample updates/s (2-D line, reference-exact GPU form)",
+        "value": updates / exact_s / 1e9, "unit": "G trace-sample updates/s", "n_gpus": 1,
+        "steps": len(ts), "warmup": 1, "ms_per_step": exact_s * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2-D line 64 CMPs x 32 offsets (2,048 traces x 1,000 samples), "
+                               "64 x 1,000 image, no aperture", "updates": updates},
+        "bit_identical_to_reference_loop": bit,
+        "fast_linear_G_updates_per_s": updates / fast_s / 1e9,
+        "cpu_baseline": {"value": updates / min(cpu) / 1e9, "unit": "G trace-sample updates/s",
+                         "cores": cores, "kind": "port",
+                         "sample": "full 2-D line, best of 3, race-free restatement"},
+        "cpu_racy_as_written_G_updates_per_s": updates / racy / 1e9,
     }
     print(json.dumps(line), flush=True)
 

@@ -224,6 +224,33 @@ psktm_status psktm_demigrate(psktm_plan *plan, const float *image,
                              const psktm_velocity *vel, float *traces_out,
                              int accumulate, void *stream);
 
+/* Batched gathers: several independent migrations (e.g. offset classes),
+ * each with its own traces and output patch, run as ONE persistent launch
+ * whose workgroups pull (gather, tile) items from a device work queue in
+ * order of decreasing cost. All gathers share the survey's sample count and
+ * interval, the tau axis (nt, tau0, dtau) of their patches, the options and
+ * a 1-D velocity v[nt]. Traces of gather g are rows first_trace ..
+ * first_trace + n_traces - 1 of the batch survey and trace arrays; images are
+ * the patches' images concatenated in gather order. No counterpart in the
+ * reference (its migration handles one trace block per call). */
+typedef struct psktm_gather {
+    psktm_grid grid;          /* output patch of this gather */
+    int        first_trace;
+    int        n_traces;
+} psktm_gather;
+typedef struct psktm_batch psktm_batch;
+psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers,
+                                int n_gathers, const psktm_survey *survey,
+                                const psktm_options *opt, void *stream);
+psktm_status psktm_batch_destroy(psktm_batch *batch);
+psktm_status psktm_batch_updates(const psktm_batch *batch, double *updates);
+/* use_graph != 0: the launch sequence (tables, queue reset, persistent
+ * kernel) is captured into a HIP graph on the first call with a given set of
+ * pointers and stream, and replayed on later calls with the same ones. */
+psktm_status psktm_batch_migrate(psktm_batch *batch, const psktm_traces *traces,
+                                 const psktm_velocity *vel, float *images,
+                                 int use_graph, void *stream);
+
 /* Reference-exact migration on device pointers (the kernel behind
  * ktmMigrationGPU): TRUNC + IEEE, 1-D velocity v[nt], no aperture, FP32. */
 psktm_status psktm_migrate_exact(const psktm_grid *grid,

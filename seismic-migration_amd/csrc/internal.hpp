@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/psktm.h"
 
@@ -110,6 +111,7 @@ struct psktm_plan {
     int adj_batch = 0;              // traces per adjoint work item
     long long list_len = 0;
     double pairs = 0.0;             // (trace, in-aperture bin) pairs
+    std::vector<double> tile_pairs; // per tile, host copy (work estimates)
 
     // per-call workspace: scaled slowness (1D or 3D) and tau^2/dt^2
     float *slow = nullptr;

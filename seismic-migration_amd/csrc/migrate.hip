@@ -28,9 +28,7 @@
 // every tile within reach of the cell and writes each trace once. It uses the
 // same traveltime, clamp, interpolation weights and aperture bits as the
 // forward, so the pair is an exact adjoint up to FP32 rounding.
-#include "internal.hpp"
-
-#include <hip/hip_fp16.h>
+#include "fwd_body.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -38,26 +36,7 @@
 namespace psktm {
 namespace {
 
-constexpr int kGMax = 8;      // traces per forward stage (upper bound)
 constexpr int kAdjMax = 8;    // traces per adjoint work item (upper bound)
-constexpr int kTBMax = 16;
-
-struct FwdArgs {
-    int nx, ny, nt, ns;
-    float x0, dx, y0, dy, tau0, dtau;
-    int tile_x, tile_y, tiles_y, n_chunks;
-    float nsf, r2, wscale;
-    int use_ap, all_traces, ntr, interp, dtype, G, nsp, accumulate;
-    const float4 *geo;
-    const int *list;
-    const long long *tile_off;
-    const int *tile_order;
-    const void *traces;
-    const float *scale;
-    const float *slow;
-    const float *tsq;
-    float *image;
-};
 
 struct AdjArgs {
     int nx, ny, nt, ns;
@@ -76,171 +55,14 @@ struct AdjArgs {
     float *traces;
 };
 
-__device__ __forceinline__ float load_sample(const void *tr, int dtype, size_t idx, float sc)
-{
-    if (dtype == PSKTM_F32) return static_cast<const float *>(tr)[idx];
-    if (dtype == PSKTM_F16)
-        return __half2float(static_cast<const __half *>(tr)[idx]) * sc;
-    unsigned u = static_cast<const unsigned short *>(tr)[idx];
-    return __uint_as_float(u << 16) * sc;
-}
-
-// LDS pair n of a trace: (d[n], d[n+1]-d[n]) for LINEAR, (d[n], 0) for TRUNC,
-// (0, 0) outside the rule's valid range (n in [0, ns]).
-__device__ __forceinline__ float2 make_pair(const void *tr, int dtype, size_t row, int n,
-                                            int ns, int interp, float sc)
-{
-    if (interp == PSKTM_INTERP_LINEAR) {
-        if (n <= ns - 2) {
-            float d0 = load_sample(tr, dtype, row + n, sc);
-            float d1 = load_sample(tr, dtype, row + n + 1, sc);
-            return make_float2(d0, d1 - d0);
-        }
-        return make_float2(0.f, 0.f);
-    }
-    if (n >= 1 && n <= ns - 1) return make_float2(load_sample(tr, dtype, row + n, sc), 0.f);
-    return make_float2(0.f, 0.f);
-}
-
-// Distance terms and aperture bit of (trace geometry ge, bin (i, j)).
-__device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, float dx,
-                                           float y0, float dy, int use_ap, float r2,
-                                           float2 *ab)
-{
-    const float x = axis_f32(x0, dx, i), y = axis_f32(y0, dy, j);
-    const bool m = !use_ap || in_aperture_f32(ge.x, ge.y, x, y, r2);
-    const float ex = ge.x - x, ey = ge.y - y;
-    const float ax = ex - ge.z, ay = ey - ge.w, bx = ex + ge.z, by = ey + ge.w;
-    *ab = make_float2(fmaf(ax, ax, ay * ay), fmaf(bx, bx, by * by));
-    return m;
-}
-
 template <int NTH, int TB, int TL, bool V3, bool KH>
 __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 {
     extern __shared__ float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
     __shared__ unsigned s_mask[kGMax];
-
-    const int tid = threadIdx.x;
-    const int T = a.tile_order[blockIdx.x / a.n_chunks];
-    const int chunk = blockIdx.x % a.n_chunks;
-    const int TI = T / a.tiles_y, TJ = T % a.tiles_y;
-    const int i0 = TI * a.tile_x, j0 = TJ * a.tile_y;
-    const int lbase = chunk * (NTH * TL) + tid;
-
-    float tq[TL];
-    float sv[V3 ? TB : 1][TL];
-    float acc[TB][TL];
-    float cmp[KH ? TB : 1][KH ? TL : 1];   // Kahan compensation (precise mode)
-#pragma unroll
-    for (int j = 0; j < TL; ++j) {
-        const int l = lbase + j * NTH;
-        tq[j] = (l < a.nt) ? a.tsq[l] : 0.f;
-        if (!V3) sv[0][j] = (l < a.nt) ? a.slow[l] : 0.f;
-    }
-#pragma unroll
-    for (int b = 0; b < TB; ++b) {
-        const int i = i0 + b / a.tile_y, jj = j0 + b % a.tile_y;
-        const bool bin_ok = (b / a.tile_y) < a.tile_x && i < a.nx && jj < a.ny;
-#pragma unroll
-        for (int j = 0; j < TL; ++j) {
-            acc[b][j] = 0.f;
-            if (KH) cmp[KH ? b : 0][KH ? j : 0] = 0.f;
-            if (V3) {
-                const int l = lbase + j * NTH;
-                sv[V3 ? b : 0][j] = (bin_ok && l < a.nt)
-                    ? a.slow[((size_t)i * a.ny + jj) * a.nt + l] : 0.f;
-            }
-        }
-    }
-
-    const long long s0 = a.all_traces ? 0 : a.tile_off[T];
-    const long long s1 = a.all_traces ? a.ntr : a.tile_off[T + 1];
-    const int G = a.G;
-    for (long long s = s0; s < s1; s += G) {
-        // ---- fill: distance terms + aperture bits, then trace pairs -------
-        if (tid < G * TB) {
-            const int g = tid / TB, b = tid % TB;
-            const long long q = s + g;
-            float2 ab = make_float2(0.f, 0.f);
-            bool m = false;
-            if (q < s1) {
-                const int k = a.all_traces ? (int)q : a.list[q];
-                const int bi = b / a.tile_y, bj = b % a.tile_y;
-                const int i = i0 + bi, jj = j0 + bj;
-                if (bi < a.tile_x && i < a.nx && jj < a.ny)
-                    m = pair_terms(a.geo[k], i, jj, a.x0, a.dx, a.y0, a.dy, a.use_ap, a.r2, &ab);
-            }
-            s_ab[g][b] = ab;
-            const unsigned long long bal = __ballot(m);
-            if ((tid % TB) == 0) {
-                const int shift = (tid & 63) - ((tid & 63) % TB);
-                s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
-            }
-        }
-        for (int g = 0; g < G; ++g) {
-            const long long q = s + g;
-            if (q >= s1) break;
-            const int k = a.all_traces ? (int)q : a.list[q];
-            const size_t row = (size_t)k * a.ns;
-            const float sc = (a.dtype == PSKTM_F32) ? 1.f : a.scale[k];
-            float2 *P = s_pairs + g * a.nsp;
-            for (int n = tid; n <= a.ns; n += NTH)
-                P[n] = make_pair(a.traces, a.dtype, row, n, a.ns, a.interp, sc);
-        }
-        __syncthreads();
-        // ---- compute --------------------------------------------------------
-        for (int g = 0; g < G; ++g) {
-            const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
-            if (!m || s + g >= s1) continue;
-            const float2 *P = s_pairs + g * a.nsp;
-#pragma unroll
-            for (int b = 0; b < TB; ++b) {
-                if (!((m >> b) & 1u)) continue;
-                const float2 ab = s_ab[g][b];
-#pragma unroll
-                for (int j = 0; j < TL; ++j) {
-                    const float sl = sv[V3 ? b : 0][j];
-                    const float ua = fmaf(ab.x, sl, tq[j]);
-                    const float ub = fmaf(ab.y, sl, tq[j]);
-                    float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
-                    x = fminf(x, a.nsf);
-                    const int n = (int)x;
-                    const float f = __builtin_amdgcn_fractf(x);
-                    const float2 pr = P[n];
-                    const float d = fmaf(f, pr.y, pr.x);
-                    if (KH) {
-                        float &c = cmp[KH ? b : 0][KH ? j : 0];
-                        const float y = fmaf(d, __builtin_amdgcn_sqrtf(x), -c);
-                        const float t = acc[b][j] + y;
-                        c = (t - acc[b][j]) - y;
-                        acc[b][j] = t;
-                    } else {
-                        acc[b][j] = fmaf(d, __builtin_amdgcn_sqrtf(x), acc[b][j]);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // ---- epilogue: weight tau*sqrt(dt), one write per sample ---------------
-#pragma unroll
-    for (int b = 0; b < TB; ++b) {
-        const int bi = b / a.tile_y, bj = b % a.tile_y;
-        const int i = i0 + bi, jj = j0 + bj;
-        if (bi >= a.tile_x || i >= a.nx || jj >= a.ny) continue;
-#pragma unroll
-        for (int j = 0; j < TL; ++j) {
-            const int l = lbase + j * NTH;
-            if (l >= a.nt) continue;
-            const float w = axis_f32(a.tau0, a.dtau, l) * a.wscale;
-            const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
-            const float sum = KH ? acc[b][j] - cmp[KH ? b : 0][KH ? j : 0] : acc[b][j];
-            const float v = sum * w;
-            a.image[idx] = a.accumulate ? a.image[idx] + v : v;
-        }
-    }
+    fwd_tile<NTH, TB, TL, V3, KH>(a, a.tile_order[blockIdx.x / a.n_chunks],
+                                  blockIdx.x % a.n_chunks, s_pairs, s_ab, s_mask);
 }
 
 template <int NTH, int TB, int TL, bool V3, typename Acc>
@@ -374,14 +196,6 @@ __global__ void zero_kernel(float *p, size_t n)
 using FwdFn = void (*)(FwdArgs);
 using AdjFn = void (*)(AdjArgs);
 
-// Per-lane register budget for the accumulators, the Kahan compensations and
-// the 3-D slowness samples; shapes beyond it are not instantiated.
-constexpr int acc_regs(int tb, int tl, bool v3, bool kh)
-{
-    return tb * tl * ((kh ? 2 : 1) + (v3 ? 1 : 0));
-}
-constexpr int reg_budget(bool kh) { return kh ? 128 : 96; }
-
 template <int NTH, int TB, int TL, bool V3, bool KH>
 FwdFn fwd_pick()
 {
@@ -415,26 +229,6 @@ FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh)
 {
     if (nth == 256) return tb == 8 ? fwd_by_flags<256, 8>(tl, v3, kh) : fwd_by_flags<256, 16>(tl, v3, kh);
     return tb == 8 ? fwd_by_flags<512, 8>(tl, v3, kh) : fwd_by_flags<512, 16>(tl, v3, kh);
-}
-
-constexpr int kTLSet[] = {2, 3, 4, 6, 8};
-
-struct FwdShape {
-    int nth, tl, chunks;
-};
-
-FwdShape choose_shape(int nt, int tb, bool v3, bool kh)
-{
-    auto fits = [&](int tl) { return acc_regs(tb, tl, v3, kh) <= reg_budget(kh); };
-    for (int nth : {256, 512}) {
-        const int need = (nt + nth - 1) / nth;
-        for (int tl : kTLSet)
-            if (tl >= need && fits(tl)) return {nth, tl, 1};
-    }
-    int tl = 2;
-    for (int t : kTLSet)
-        if (fits(t)) tl = t;
-    return {512, tl, (nt + 512 * tl - 1) / (512 * tl)};
 }
 
 } // namespace
@@ -499,21 +293,12 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
 
     FwdArgs a{};
-    a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
-    a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;
-    a.tile_x = p->tile_x; a.tile_y = p->tile_y; a.tiles_y = p->tiles_y; a.n_chunks = sh.chunks;
-    a.nsf = (float)p->n_samples;
-    a.r2 = p->opt.aperture * p->opt.aperture;
-    a.wscale = (float)std::sqrt((double)p->dt);
-    a.use_ap = !p->all_traces;
-    a.all_traces = p->all_traces;
-    a.ntr = p->n_traces;
-    a.interp = p->opt.interp;
+    fwd_args_from_plan(p, a);
+    a.n_chunks = sh.chunks;
     a.dtype = tr.dtype;
     a.G = G;
     a.nsp = nsp;
     a.accumulate = accumulate;
-    a.geo = p->geo; a.list = p->list; a.tile_off = p->tile_off; a.tile_order = p->tile_order;
     a.traces = tr.data; a.scale = tr.scale; a.slow = p->slow; a.tsq = p->tsq; a.image = image;
     const size_t lds = (size_t)G * slot;
     { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }

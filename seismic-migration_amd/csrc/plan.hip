@@ -367,6 +367,7 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
         pairs += (double)h_pairs[T];
     }
     p->pairs = pairs;
+    p->tile_pairs.assign(h_pairs.begin(), h_pairs.end());
     p->list_len = p->all_traces ? 0 : h_off[nT];
     // launch order: decreasing (traces x bins) so that long tiles start first
     std::vector<int> order(nT);

@@ -64,6 +64,10 @@ class PsktmOptions(ctypes.Structure):
                 ("precise", _c_int)]
 
 
+class PsktmGather(ctypes.Structure):
+    _fields_ = [("grid", PsktmGrid), ("first_trace", _c_int), ("n_traces", _c_int)]
+
+
 _P = ctypes.POINTER
 _SIGNATURES = {
     # reference API
@@ -91,6 +95,11 @@ _SIGNATURES = {
     "psktm_plan_bytes": (_c_int, [_p, _P(_c_size)]),
     "psktm_migrate": (_c_int, [_p, _P(PsktmTraces), _P(PsktmVelocity), _p, _c_int, _p]),
     "psktm_demigrate": (_c_int, [_p, _p, _P(PsktmVelocity), _p, _c_int, _p]),
+    "psktm_batch_create": (_c_int, [_P(_p), _P(PsktmGather), _c_int, _P(PsktmSurvey),
+                                    _P(PsktmOptions), _p]),
+    "psktm_batch_destroy": (_c_int, [_p]),
+    "psktm_batch_updates": (_c_int, [_p, _P(_c_double)]),
+    "psktm_batch_migrate": (_c_int, [_p, _P(PsktmTraces), _P(PsktmVelocity), _p, _c_int, _p]),
     "psktm_migrate_exact": (_c_int, [_P(PsktmGrid), _P(PsktmSurvey), _p, _p, _p, _p]),
     "psktm_rho_filter": (_c_int, [_p, _c_int, _c_int, _c_float, _c_int, _c_int, _p, _p, _c_int, _p]),
     "psktm_quantize": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p, _p]),

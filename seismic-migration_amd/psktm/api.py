@@ -18,7 +18,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (F32, INTERP_LINEAR, MATH_FAST, VEL_1D, VEL_3D, ImageGrid, JobParams,
-                   PsktmGrid, PsktmOptions, PsktmSurvey, PsktmTraces, PsktmVelocity, check)
+                   PsktmGather, PsktmGrid, PsktmOptions, PsktmSurvey, PsktmTraces,
+                   PsktmVelocity, check)
 
 _p = ctypes.c_void_p
 
@@ -153,6 +154,74 @@ class Plan:
     def close(self):
         if self._h:
             _lib.load().psktm_plan_destroy(self._h)
+            self._h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """psktm_batch: independent gathers migrated by one persistent launch.
+
+    ``gathers`` is a list of (Grid, first_trace, n_traces); traces of gather g
+    are rows first_trace .. first_trace+n_traces-1 of the survey and trace
+    arrays. Images are returned concatenated in gather order.
+    """
+
+    def __init__(self, gathers, survey: Survey, *, interp=INTERP_LINEAR, aperture=0.0,
+                 tile_x=0, tile_y=0, traces_per_stage=0, precise=False, stream=None):
+        L = _lib.load()
+        self.gathers = [(g, int(f), int(n)) for g, f, n in gathers]
+        self.survey = survey
+        arr = (PsktmGather * len(self.gathers))()
+        for q, (g, f, n) in enumerate(self.gathers):
+            arr[q] = PsktmGather(g.c(), f, n)
+        self.opt = PsktmOptions(int(interp), int(MATH_FAST), float(aperture), int(tile_x),
+                                int(tile_y), int(traces_per_stage), int(bool(precise)))
+        self._h = _p()
+        s = survey.c()
+        check(L.psktm_batch_create(ctypes.byref(self._h), arr, len(self.gathers),
+                                   ctypes.byref(s), ctypes.byref(self.opt), _stream(stream)))
+        self.sizes = [g.nx * g.ny * g.nt for g, _, _ in self.gathers]
+        self.offsets = list(np.cumsum([0] + self.sizes[:-1]))
+        self.nt = self.gathers[0][0].nt
+
+    def updates(self) -> float:
+        v = ctypes.c_double()
+        check(_lib.load().psktm_batch_updates(self._h, ctypes.byref(v)))
+        return v.value
+
+    def migrate(self, traces, vel, images=None, *, scale=None, use_graph=False, stream=None):
+        torch = _torch()
+        L = _lib.load()
+        kinds = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
+        _need(traces, traces.dtype, "traces", self.survey.n_traces * self.survey.n_samples)
+        kind = kinds[traces.dtype]
+        if kind != _lib.F32:
+            _need(scale, torch.float32, "scale", self.survey.n_traces)
+        _need(vel, torch.float32, "vel", self.nt)
+        total = int(sum(self.sizes))
+        if images is None:
+            images = torch.empty(total, dtype=torch.float32, device=traces.device)
+        _need(images, torch.float32, "images", total)
+        tr = PsktmTraces(kind, _ptr(traces), _ptr(scale) if kind != _lib.F32 else 0)
+        v = PsktmVelocity(VEL_1D, _ptr(vel))
+        check(L.psktm_batch_migrate(self._h, ctypes.byref(tr), ctypes.byref(v), _p(_ptr(images)),
+                                    int(bool(use_graph)), _stream(stream)))
+        return images
+
+    def image(self, images, g):
+        """View of gather g's patch image [nx, ny, nt] inside ``images``."""
+        grid = self.gathers[g][0]
+        o = int(self.offsets[g])
+        return images[o:o + self.sizes[g]].view(grid.nx, grid.ny, grid.nt)
+
+    def close(self):
+        if self._h:
+            _lib.load().psktm_batch_destroy(self._h)
             self._h = _p()
 
     def __del__(self):

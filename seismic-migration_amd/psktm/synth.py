@@ -12,6 +12,8 @@ has dtau = d/2 (src/psktmCPU.c:78-79); velocities are RMS velocities in m/s.
   uniform 0-4000 m, uniform azimuths, v(x,y,t0) = 1500 + 600 t0 +
   200 sin(x/2000) cos(y/3000), 30 Hz Ricker point diffractors at DSR times
   plus Gaussian noise. Built with torch on any device (GPU at full size).
+* :func:`offset_class_gathers` - 64 offset-class gathers with lognormal trace
+  counts and square patches of 32-128 bins, for the batched launch.
 """
 from __future__ import annotations
 
@@ -118,13 +120,34 @@ def diffractor_3d(n_traces: int, *, nx=200, ny=200, dxy=25.0, nt=1500, dtau=0.00
     noise_seed = int(torch.randint(0, 2**31 - 1, (1,), generator=g).item())
     dev = torch.device(device)
     mx32, my32, hx32, hy32 = (a.to(torch.float32) for a in (mx, my, hx, hy))
-    # positions the operator sees (from the float32 mid/off)
+    vd = velocity_3d_fn(dxp, dyp, 2 * dtau_d)
+    traces = diffraction_traces(mx32, my32, hx32, hy32, dxp, dyp, dtau_d, damp, vd, ns, dt,
+                                f_peak, noise_db, noise_seed, dev, chunk)
+    xs = torch.arange(nx, dtype=torch.float32) * np.float32(dxy)
+    ys = torch.arange(ny, dtype=torch.float32) * np.float32(dxy)
+    taus = torch.arange(nt, dtype=torch.float32) * np.float32(dtau)
+    vel = velocity_3d_fn(xs[:, None, None].double(), ys[None, :, None].double(),
+                         2.0 * taus[None, None, :].double()).to(torch.float32).to(dev).contiguous()
+    grid = (nx, 0.0, dxy, ny, 0.0, dxy, nt, 0.0, dtau)
+    return Survey3D(grid, mx32.to(dev), my32.to(dev), hx32.to(dev), hy32.to(dev), traces, vel,
+                    dt, aperture, seed)
+
+
+def diffraction_traces(mx32, my32, hx32, hy32, dxp, dyp, dtau_d, damp, vd, ns, dt, f_peak,
+                       noise_db, noise_seed, dev, chunk=16384):
+    """Point diffractors (Ricker wavelets at DSR times) plus Gaussian noise.
+
+    Positions are the ones the operator sees, s = mid - off, r = mid + off,
+    formed in FP64 from the float32 midpoints and half offsets; diffractor d
+    sits at (dxp, dyp), one-way time dtau_d, RMS velocity vd, amplitude damp.
+    """
+    import torch
+    n_traces = mx32.numel()
     sx = (mx32.double() - hx32.double()).to(dev)
     sy = (my32.double() - hy32.double()).to(dev)
     rx = (mx32.double() + hx32.double()).to(dev)
     ry = (my32.double() + hy32.double()).to(dev)
-    vd = velocity_3d_fn(dxp, dyp, 2 * dtau_d).to(dev)
-    dxp, dyp, dtau_d, damp = dxp.to(dev), dyp.to(dev), dtau_d.to(dev), damp.to(dev)
+    dxp, dyp, dtau_d, damp, vd = (a.to(dev) for a in (dxp, dyp, dtau_d, damp, vd))
     traces = torch.empty((n_traces, ns), dtype=torch.float32, device=dev)
     half = int(math.ceil(2.5 / (math.pi * f_peak) / dt)) + 1     # wavelet support
     win = torch.arange(-half, half + 1, device=dev, dtype=torch.float64)
@@ -146,17 +169,82 @@ def diffractor_3d(n_traces: int, *, nx=200, ny=200, dxy=25.0, nt=1500, dtau=0.00
         blk = torch.zeros((b - a) * ns, dtype=torch.float64, device=dev)
         blk.index_add_(0, flat, val[ok])
         traces[a:b] = blk.view(b - a, ns).to(torch.float32)
-    rms = torch.sqrt(torch.mean(traces.double() ** 2)).item()
-    sigma = rms / (10 ** (noise_db / 20.0))
-    ng = torch.Generator(device=dev).manual_seed(noise_seed)
-    for a in range(0, n_traces, chunk * 8):
-        b = min(n_traces, a + chunk * 8)
-        traces[a:b] += sigma * torch.randn((b - a, ns), generator=ng, device=dev, dtype=torch.float32)
-    xs = torch.arange(nx, dtype=torch.float32) * np.float32(dxy)
-    ys = torch.arange(ny, dtype=torch.float32) * np.float32(dxy)
-    taus = torch.arange(nt, dtype=torch.float32) * np.float32(dtau)
-    vel = velocity_3d_fn(xs[:, None, None].double(), ys[None, :, None].double(),
-                         2.0 * taus[None, None, :].double()).to(torch.float32).to(dev).contiguous()
-    grid = (nx, 0.0, dxy, ny, 0.0, dxy, nt, 0.0, dtau)
-    return Survey3D(grid, mx32.to(dev), my32.to(dev), hx32.to(dev), hy32.to(dev), traces, vel,
-                    dt, aperture, seed)
+    if n_traces:
+        rms = torch.sqrt(torch.mean(traces.double() ** 2)).item()
+        sigma = rms / (10 ** (noise_db / 20.0))
+        ng = torch.Generator(device=dev).manual_seed(noise_seed)
+        for a in range(0, n_traces, chunk * 8):
+            b = min(n_traces, a + chunk * 8)
+            traces[a:b] += sigma * torch.randn((b - a, ns), generator=ng, device=dev,
+                                               dtype=torch.float32)
+    return traces
+
+
+@dataclass
+class Gathers:
+    grids: list          # per gather (nx, x0, dx, ny, y0, dy, nt, tau0, dtau)
+    first: list          # first trace of each gather in the concatenated arrays
+    counts: list
+    mid_x: object
+    mid_y: object
+    off_x: object
+    off_y: object
+    traces: object       # torch [sum counts, ns] float32
+    vel: object          # torch [nt] float32, v_rms(t0 = 2 tau)
+    dt: float
+    aperture: float
+    seed: int
+    params: dict
+
+
+def offset_class_gathers(n_gathers: int = 64, *, mu=math.log(11000.0), sigma=0.8,
+                         n_min=2000, n_max=60000, side_min=32, side_max=128, dxy=25.0,
+                         nt=1000, dtau=0.002, ns=1000, dt=0.004, class_width=62.5,
+                         aperture=1500.0, n_diff=4, f_peak=30.0, noise_db=20.0,
+                         count_scale=1.0, seed=SEED0 + 3, device="cpu") -> Gathers:
+    """Offset-class gathers: gather g holds traces whose full offset lies in
+    [class_width*g, class_width*(g+1)); trace counts lognormal(mu, sigma)
+    clipped to [n_min, n_max] (times count_scale); square patches with a side
+    uniform in [side_min, side_max] bins; midpoints uniform over the patch;
+    v_rms(t0) = 1500 + 600 t0 shared by all gathers."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    grids, first, counts = [], [], []
+    mids_x, mids_y, offs_x, offs_y, trs = [], [], [], [], []
+    vel_fn = lambda t0: 1500.0 + 600.0 * t0
+    total = 0
+    for k in range(n_gathers):
+        n = float(torch.exp(mu + sigma * torch.randn(1, generator=g, dtype=torch.float64)))
+        n = int(max(1, round(min(max(n, n_min), n_max) * count_scale)))
+        side = int(torch.randint(side_min, side_max + 1, (1,), generator=g))
+        ox = float(torch.randint(0, 400, (1,), generator=g)) * dxy
+        oy = float(torch.randint(0, 400, (1,), generator=g)) * dxy
+        ext = (side - 1) * dxy
+        mx = ox + torch.rand(n, generator=g, dtype=torch.float64) * ext
+        my = oy + torch.rand(n, generator=g, dtype=torch.float64) * ext
+        off = (k + torch.rand(n, generator=g, dtype=torch.float64)) * class_width
+        az = torch.rand(n, generator=g, dtype=torch.float64) * (2 * math.pi)
+        hx, hy = 0.5 * off * torch.cos(az), 0.5 * off * torch.sin(az)
+        dxp = ox + torch.rand(n_diff, generator=g, dtype=torch.float64) * ext
+        dyp = oy + torch.rand(n_diff, generator=g, dtype=torch.float64) * ext
+        tmax = (ns - 1) * dt
+        dtd = 0.1 * tmax + torch.rand(n_diff, generator=g, dtype=torch.float64) * 0.3 * tmax
+        damp = torch.where(torch.rand(n_diff, generator=g) < 0.5, -1.0, 1.0).to(torch.float64)
+        noise_seed = int(torch.randint(0, 2**31 - 1, (1,), generator=g))
+        m32 = [a.to(torch.float32) for a in (mx, my, hx, hy)]
+        tr = diffraction_traces(*m32, dxp, dyp, dtd, damp, vel_fn(2 * dtd), ns, dt, f_peak,
+                                noise_db, noise_seed, torch.device(device))
+        grids.append((side, ox, dxy, side, oy, dxy, nt, 0.0, dtau))
+        first.append(total)
+        counts.append(n)
+        total += n
+        mids_x.append(m32[0]); mids_y.append(m32[1]); offs_x.append(m32[2]); offs_y.append(m32[3])
+        trs.append(tr)
+    dev = torch.device(device)
+    cat = lambda xs: torch.cat(xs).to(dev).contiguous()
+    tau = torch.arange(nt, dtype=torch.float32) * np.float32(dtau)
+    vel = vel_fn(2.0 * tau.double()).to(torch.float32).to(dev)
+    return Gathers(grids, first, counts, cat(mids_x), cat(mids_y), cat(offs_x), cat(offs_y),
+                   torch.cat(trs).contiguous(), vel, dt, aperture, seed,
+                   dict(mu=mu, sigma=sigma, n_min=n_min, n_max=n_max, side_min=side_min,
+                        side_max=side_max, class_width=class_width, count_scale=count_scale))
