@@ -245,8 +245,9 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
 psktm_status psktm_batch_destroy(psktm_batch *batch);
 psktm_status psktm_batch_updates(const psktm_batch *batch, double *updates);
 /* use_graph != 0: the launch sequence (tables, queue reset, persistent
- * kernel) is captured into a HIP graph on the first call with a given set of
- * pointers and stream, and replayed on later calls with the same ones. */
+ * kernel) is recorded into a HIP graph on the first call with a given set of
+ * pointers and replayed, on the caller's stream, by later calls with the same
+ * ones. A batch serves one stream at a time. */
 psktm_status psktm_batch_migrate(psktm_batch *batch, const psktm_traces *traces,
                                  const psktm_velocity *vel, float *images,
                                  int use_graph, void *stream);
@@ -290,6 +291,39 @@ psktm_status psktm_load_ktm_config(const char *path, struct imageGrid *imageX,
                                    struct imageGrid *imageY,
                                    struct imageGrid *imageZ,
                                    struct jobParams *config);
+
+/* Trace preparation on the device (the MATLAB writers' steps).
+ *
+ * Offset sort (scripts/ktmGetSortedOffTraces.m:17-22): perm[i] = index of
+ * the trace with the i-th smallest half-offset magnitude sqrt(hx^2 + hy^2),
+ * ties in trace order (stable); key[i] = that magnitude (may be NULL). */
+psktm_status psktm_offset_sort(const float *off_x, const float *off_y, int n,
+                               int *perm, float *key, void *stream);
+/* Offset classes: class c holds traces with full offset 2|h| in
+ * [c*width, (c+1)*width), c < n_classes (larger offsets are dropped).
+ * perm lists the kept traces class by class (trace order within a class);
+ * class_start[c] .. class_start[c+1] is class c's range (n_classes + 1
+ * entries). Returns the number kept in *n_kept (host). */
+psktm_status psktm_offset_classes(const float *off_x, const float *off_y, int n,
+                                  float width, int n_classes, int *perm,
+                                  int *class_start, int *n_kept, void *stream);
+/* Constant-offset selection (scripts/ktmGetConstOffTraces.m:18-28): keeps
+ * traces with h - tol < round(|h|) < h + tol, in trace order; *n_kept on
+ * the host. */
+psktm_status psktm_select_offset(const float *off_x, const float *off_y, int n,
+                                 float h, float tol, int *selected, int *n_kept,
+                                 void *stream);
+/* Gather + zero padding (scripts/ktmWriteTraceBlockConstOffset.m:100-118):
+ * out[r][s] = in[rows[r]][s] for r < n_rows, s < m; zero elsewhere, with
+ * out of shape [ceil(n_rows/sze)*sze][ceil(m/sze)*sze] (rows NULL means
+ * identity). */
+psktm_status psktm_gather_pad(const float *in, int m, const int *rows, int n_rows,
+                              int sze, float *out, void *stream);
+/* Temporal upsampling by 2^N (scripts/ktmInterpTraceBlock.m:134-154): each
+ * trace of m samples becomes (m-1)*2^N + 1 samples by linear interpolation
+ * on the sample grid 0 : 2^-N : m-1. */
+psktm_status psktm_upsample_traces(const float *in, int n, int m, int N, float *out,
+                                   void *stream);
 
 /* FP64 reductions for conjugate gradients, stream-ordered, no host sync:
  * result[0] = sum_i a[i]*b[i] accumulated in double. */

@@ -231,3 +231,43 @@ def rho_filter_per_trace(trace, dw, opt):
     trace = np.asarray(trace, dtype=np.float64)
     return np.concatenate([apply_rho_filter(trace[c:c + 1], dw, opt) for c in range(trace.shape[0])], 0) \
         if trace.shape[0] else trace.copy()
+
+
+# --------------------------------------------------------------------------
+# Trace preparation: restatements of the MATLAB writers' steps.
+# --------------------------------------------------------------------------
+
+def sorted_offset_order(off_x, off_y):
+    """[~, index] = sort(offset) (scripts/ktmGetSortedOffTraces.m:17-19): MATLAB's
+    sort is stable; offset is the half-offset magnitude."""
+    mag = np.sqrt(np.asarray(off_x, np.float64) ** 2 + np.asarray(off_y, np.float64) ** 2)
+    return np.argsort(mag, kind="stable")
+
+
+def const_offset_selection(off_x, off_y, h, tol):
+    """find((offset > h - tol) & (offset < h + tol)) with offset =
+    round(|h|) (scripts/ktmGetConstOffTraces.m:18-23; MATLAB round: halves
+    away from zero)."""
+    mag = np.sqrt(np.asarray(off_x, np.float64) ** 2 + np.asarray(off_y, np.float64) ** 2)
+    r = np.floor(mag + 0.5)
+    return np.nonzero((r > h - tol) & (r < h + tol))[0]
+
+
+def pad_block(traces, sze):
+    """Zero padding of records and samples to multiples of sze
+    (scripts/ktmWriteTraceBlockConstOffset.m:100-118)."""
+    traces = np.asarray(traces)
+    n, m = traces.shape
+    out = np.zeros((-(-n // sze) * sze, -(-m // sze) * sze), traces.dtype)
+    out[:n, :m] = traces
+    return out
+
+
+def upsample_time(traces, N):
+    """interp1(0:NN-1, trace, 0:1/2^N:NN-1) for every trace
+    (scripts/ktmInterpTraceBlock.m:134-154)."""
+    traces = np.asarray(traces, np.float64)
+    n, m = traces.shape
+    t_new = np.arange((m - 1) * 2 ** N + 1) / 2.0 ** N
+    return np.stack([np.interp(t_new, np.arange(m), traces[k]) for k in range(n)]) \
+        if n else np.zeros((0, len(t_new)))

@@ -117,7 +117,8 @@ struct psktm_batch {
     double pairs = 0.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    const void *key[6] = {};
+    hipStream_t cap = nullptr;       // private stream used only to record the graph
+    const void *key[5] = {};
 };
 
 namespace psktm {
@@ -128,6 +129,7 @@ void batch_free(psktm_batch *b)
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
     for (psktm_plan *p : b->plans)
         if (p) psktm_plan_destroy(p);
     void *bufs[] = {b->d_items, b->d_counter, b->d_args, b->slow, b->tsq};
@@ -310,9 +312,9 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
     PSKTM_CHECK(tr->data && (tr->dtype == PSKTM_F32 || tr->scale), PSKTM_E_INVAL,
                 "trace data or scales missing");
     hipStream_t st = as_stream(stream);
-    const void *key[6] = {tr->data, tr->scale, vel->v, images, stream,
+    const void *key[5] = {tr->data, tr->scale, vel->v, images,
                           (const void *)(intptr_t)(tr->dtype + 1)};
-    const bool same = b->exec && std::equal(key, key + 6, b->key);
+    const bool same = b->exec && std::equal(key, key + 5, b->key);
     if (use_graph && same) {
         PSKTM_HIP(hipGraphLaunch(b->exec, st));
         return PSKTM_OK;
@@ -342,16 +344,20 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
         PSKTM_HIP(hipStreamSynchronize(st));
         return enqueue(b, vel->v, st);
     }
-    // (re)capture for these pointers
+    // (re)capture for these pointers. Recording happens on a private
+    // non-blocking stream (the caller's stream may be the legacy default
+    // stream, which cannot be captured); the graph is launched on the
+    // caller's stream, so it stays ordered with the caller's work.
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    if (!b->cap) PSKTM_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
     PSKTM_HIP(hipStreamSynchronize(st));
     PSKTM_HIP(hipMemcpy(b->d_args, b->h_args.data(), sizeof(FwdArgs) * b->h_args.size(),
                         hipMemcpyHostToDevice));
-    PSKTM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    psktm_status s = enqueue(b, vel->v, st);
+    PSKTM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    psktm_status s = enqueue(b, vel->v, b->cap);
     hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &graph);
+    hipError_t e = hipStreamEndCapture(b->cap, &graph);
     if (s != PSKTM_OK) {
         if (graph) (void)hipGraphDestroy(graph);
         return s;
@@ -359,7 +365,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
     b->graph = graph;
     PSKTM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    std::copy(key, key + 6, b->key);
+    std::copy(key, key + 5, b->key);
     PSKTM_HIP(hipGraphLaunch(b->exec, st));
     return PSKTM_OK;
 }

@@ -107,6 +107,11 @@ _SIGNATURES = {
     "psktm_ibm32_to_ieee_device": (_c_int, [_p, _p, _c_size, _p]),
     "psktm_load_ktm_config": (_c_int, [ctypes.c_char_p, _P(ImageGrid), _P(ImageGrid),
                                        _P(ImageGrid), _P(JobParams)]),
+    "psktm_offset_sort": (_c_int, [_p, _p, _c_int, _p, _p, _p]),
+    "psktm_offset_classes": (_c_int, [_p, _p, _c_int, _c_float, _c_int, _p, _p, _P(_c_int), _p]),
+    "psktm_select_offset": (_c_int, [_p, _p, _c_int, _c_float, _c_float, _p, _P(_c_int), _p]),
+    "psktm_gather_pad": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _p, _p]),
+    "psktm_upsample_traces": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p]),
     "psktm_dot_f64": (_c_int, [_p, _p, _c_size, _p, _p]),
     "psktm_axpy_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _c_float, _p]),
     "psktm_xpby_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _p]),

@@ -323,6 +323,83 @@ def dot_f64(a, b, out=None, *, stream=None):
 
 
 # --------------------------------------------------------------------------
+# Trace preparation (the MATLAB writers' steps), device tensors.
+# --------------------------------------------------------------------------
+
+def offset_sort(off_x, off_y, *, stream=None):
+    """Permutation sorting traces by half-offset magnitude (stable), and the
+    sorted magnitudes (scripts/ktmGetSortedOffTraces.m:17-22)."""
+    torch = _torch()
+    n = off_x.numel()
+    _need(off_x, torch.float32, "off_x")
+    _need(off_y, torch.float32, "off_y", n)
+    perm = torch.empty(n, dtype=torch.int32, device=off_x.device)
+    key = torch.empty(n, dtype=torch.float32, device=off_x.device)
+    check(_lib.load().psktm_offset_sort(_p(_ptr(off_x)), _p(_ptr(off_y)), n, _p(_ptr(perm)),
+                                        _p(_ptr(key)), _stream(stream)))
+    return perm, key
+
+
+def offset_classes(off_x, off_y, width, n_classes, *, stream=None):
+    """(perm of kept traces class by class, class_start[n_classes+1])."""
+    torch = _torch()
+    n = off_x.numel()
+    _need(off_x, torch.float32, "off_x")
+    _need(off_y, torch.float32, "off_y", n)
+    perm = torch.empty(max(n, 1), dtype=torch.int32, device=off_x.device)
+    start = torch.empty(n_classes + 1, dtype=torch.int32, device=off_x.device)
+    kept = ctypes.c_int()
+    check(_lib.load().psktm_offset_classes(_p(_ptr(off_x)), _p(_ptr(off_y)), n, float(width),
+                                           int(n_classes), _p(_ptr(perm)), _p(_ptr(start)),
+                                           ctypes.byref(kept), _stream(stream)))
+    return perm[:kept.value], start
+
+
+def select_offset(off_x, off_y, h, tol, *, stream=None):
+    """Indices of traces with h - tol < round(|h|) < h + tol, in trace order
+    (scripts/ktmGetConstOffTraces.m:18-28)."""
+    torch = _torch()
+    n = off_x.numel()
+    _need(off_x, torch.float32, "off_x")
+    _need(off_y, torch.float32, "off_y", n)
+    sel = torch.empty(max(n, 1), dtype=torch.int32, device=off_x.device)
+    kept = ctypes.c_int()
+    check(_lib.load().psktm_select_offset(_p(_ptr(off_x)), _p(_ptr(off_y)), n, float(h),
+                                          float(tol), _p(_ptr(sel)), ctypes.byref(kept),
+                                          _stream(stream)))
+    return sel[:kept.value]
+
+
+def gather_pad(traces, sze, rows=None, *, stream=None):
+    """Rows ``rows`` of ``traces`` (all if None), zero padded in both
+    dimensions to multiples of ``sze`` (scripts/ktmWriteTraceBlockConstOffset.m:100-118)."""
+    torch = _torch()
+    _need(traces, torch.float32, "traces")
+    n, m = traces.shape
+    nr = n if rows is None else rows.numel()
+    if rows is not None:
+        _need(rows, torch.int32, "rows")
+    R, C = -(-nr // sze) * sze, -(-m // sze) * sze
+    out = torch.empty((R, C), dtype=torch.float32, device=traces.device)
+    check(_lib.load().psktm_gather_pad(_p(_ptr(traces)), m, _p(_ptr(rows)), nr, int(sze),
+                                       _p(_ptr(out)), _stream(stream)))
+    return out
+
+
+def upsample_traces(traces, N, *, stream=None):
+    """Linear interpolation of every trace onto the sample grid
+    0 : 2^-N : m-1 (scripts/ktmInterpTraceBlock.m:134-154)."""
+    torch = _torch()
+    _need(traces, torch.float32, "traces")
+    n, m = traces.shape
+    mo = (m - 1) * (1 << N) + 1
+    out = torch.empty((n, mo), dtype=torch.float32, device=traces.device)
+    check(_lib.load().psktm_upsample_traces(_p(_ptr(traces)), n, m, int(N), _p(_ptr(out)),
+                                            _stream(stream)))
+    return out
+
+
+# --------------------------------------------------------------------------
 # Reference-named host entry points.
 # --------------------------------------------------------------------------
 

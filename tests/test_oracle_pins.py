@@ -319,3 +319,22 @@ def test_rho_matlab_fftshift_rotates_odd_trace_counts():
     x4 = rng.standard_normal((4, 64))
     np.testing.assert_allclose(oracle.apply_rho_filter(x4, 4000.0, 2),
                                oracle.rho_filter_per_trace(x4, 4000.0, 2), rtol=1e-12, atol=1e-9)
+
+
+# ---------------------------- trace preparation ---------------------------
+
+def test_prep_restatements_closed_forms():
+    # sort: magnitudes 5, 5, 1, 13 -> stable order 2, 0, 1, 3
+    hx = np.array([3.0, 4.0, 1.0, 5.0]); hy = np.array([4.0, 3.0, 0.0, 12.0])
+    assert list(oracle.sorted_offset_order(hx, hy)) == [2, 0, 1, 3]
+    # selection: round(2.5) = 3 (halves away from zero), window open at both ends
+    hx = np.array([2.5, 2.4, 7.0, 10.0]); hy = np.zeros(4)
+    assert list(oracle.const_offset_selection(hx, hy, 3.0, 1.0)) == [0]
+    assert list(oracle.const_offset_selection(hx, hy, 8.5, 2.0)) == [2, 3]
+    # padding
+    p = oracle.pad_block(np.ones((3, 5)), 4)
+    assert p.shape == (4, 8) and p.sum() == 15 and p[:3, :5].all()
+    # upsampling of a linear ramp is exact, length (m-1)*2^N + 1
+    r = np.arange(6.0)[None] * 2 + 1
+    u = oracle.upsample_time(r, 2)
+    assert u.shape == (1, 21) and np.allclose(u[0], 1 + 2 * np.arange(21) / 4.0)
