@@ -217,7 +217,7 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
         psktm__set_error("no batch kernel for tile %d", b->tb);
         return PSKTM_E_UNSUPPORTED;
     }
-    b->nsp = (b->ns + 2) & ~1;
+    b->nsp = pair_stride(b->ns);
     const size_t slot = (size_t)b->nsp * sizeof(float2);
     int G = opt->traces_per_stage > 0 ? opt->traces_per_stage : 4;
     G = std::min(G, kGMax);

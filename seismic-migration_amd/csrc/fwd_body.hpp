@@ -31,30 +31,76 @@ struct FwdArgs {
 };
 
 
-__device__ __forceinline__ float load_sample(const void *tr, int dtype, size_t idx, float sc)
+// LDS pair table of one staged trace: entry n holds (d[n], d[n+1]-d[n]) for
+// LINEAR (valid n in [0, ns-2]) or (d[n], 0) for TRUNC (valid n in [1, ns-1]),
+// and (0, 0) elsewhere; d = stored sample x per-trace scale. A row keeps
+// kPairFront entries in front of n = 0 and enough behind n = ns that the
+// 16-byte chunk writes of the fill never leave the row.
+constexpr int kPairFront = 8;
+__host__ __device__ inline int pair_stride(int ns) { return (ns + 2 * kPairFront + 1) & ~1; }
+
+__device__ __forceinline__ float decode16(unsigned bits, int dtype, float sc)
 {
-    if (dtype == PSKTM_F32) return static_cast<const float *>(tr)[idx];
-    if (dtype == PSKTM_F16)
-        return __half2float(static_cast<const __half *>(tr)[idx]) * sc;
-    unsigned u = static_cast<const unsigned short *>(tr)[idx];
-    return __uint_as_float(u << 16) * sc;
+    if (dtype == PSKTM_F16) return __half2float(__ushort_as_half((unsigned short)bits)) * sc;
+    return __uint_as_float(bits << 16) * sc;    // BF16
 }
 
-// LDS pair n of a trace: (d[n], d[n+1]-d[n]) for LINEAR, (d[n], 0) for TRUNC,
-// (0, 0) outside the rule's valid range (n in [0, ns]).
-__device__ __forceinline__ float2 make_pair(const void *tr, int dtype, size_t row, int n,
-                                            int ns, int interp, float sc)
+// Stages the trace rows s .. s+G-1 of the tile list into the LDS pair table.
+// Work is split into (trace, group of 64 chunks) items, one item per wave at
+// a time, so the row address is wave-uniform and each row is read through a
+// buffer descriptor whose range ends at the row's last byte. Each lane loads
+// one 16-byte chunk (8 FP16/BF16 or 4 FP32 samples, from a 16-byte aligned
+// base) and the first sample of the next chunk, then writes one pair per
+// sample. A load whose start lies beyond the range returns zeros; a chunk
+// whose start lies inside it stays inside one 16-byte block that holds a byte
+// of the row, so no access leaves the pages of the trace buffer. Samples
+// outside [0, ns] land in the row's padding and are never read.
+template <int NTH>
+__device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long long s1,
+                                           float2 *__restrict__ s_pairs)
 {
-    if (interp == PSKTM_INTERP_LINEAR) {
-        if (n <= ns - 2) {
-            float d0 = load_sample(tr, dtype, row + n, sc);
-            float d1 = load_sample(tr, dtype, row + n + 1, sc);
-            return make_float2(d0, d1 - d0);
+    constexpr int W = NTH / 64;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int E = a.dtype == PSKTM_F32 ? 4 : 2;
+    const int per = 16 / E;
+    const int G = (int)((s1 - s) < (long long)a.G ? (s1 - s) : (long long)a.G);
+    const int groups = ((14 + a.ns * E) / 16 + 1 + 63) / 64;
+    const bool lin = a.interp == PSKTM_INTERP_LINEAR;
+    for (int it = wave; it < G * groups; it += W) {
+        const int g = it / groups, cg = it - g * groups;
+        const long long q = s + g;
+        const int k = a.all_traces ? (int)q : a.list[q];
+        const uintptr_t ra = (uintptr_t)a.traces + (size_t)k * a.ns * E;
+        const int sb = (int)(ra & 15u);
+        const int n_chunks = (sb + a.ns * E) / 16 + 1;
+        const int c = cg * 64 + lane;
+        if (c >= n_chunks) continue;
+        const float sc = a.dtype == PSKTM_F32 ? 1.f : a.scale[k];
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(ra & ~(uintptr_t)15), (short)0, sb + a.ns * E, 0x00020000);
+        const int n0 = (16 * c - sb) / E;           // first sample of the chunk
+        float2 *P = s_pairs + g * a.nsp + kPairFront + n0;
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * c, 0, 0);
+        const unsigned nx = E == 4 ? __builtin_amdgcn_raw_buffer_load_b32(rs, 16 * c + 16, 0, 0)
+                                   : __builtin_amdgcn_raw_buffer_load_b16(rs, 16 * c + 16, 0, 0);
+        // sample i of the chunk (i = per is the next chunk's first sample)
+        auto sample = [&](int i) -> float {
+            if (E == 4) return __uint_as_float(i < 4 ? v[i] : nx);
+            const unsigned w = i < 8 ? ((v[i >> 1] >> (16 * (i & 1))) & 0xffffu) : nx;
+            return decode16(w, a.dtype, sc);
+        };
+        float cur = sample(0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i >= per) break;
+            const float nxt = sample(i + 1);
+            const int n = n0 + i;
+            const bool ok = lin ? (n >= 0 && n <= a.ns - 2) : (n >= 1 && n <= a.ns - 1);
+            P[i] = ok ? make_float2(cur, lin ? nxt - cur : 0.f) : make_float2(0.f, 0.f);
+            cur = nxt;
         }
-        return make_float2(0.f, 0.f);
     }
-    if (n >= 1 && n <= ns - 1) return make_float2(load_sample(tr, dtype, row + n, sc), 0.f);
-    return make_float2(0.f, 0.f);
 }
 
 // Distance terms and aperture bit of (trace geometry ge, bin (i, j)).
@@ -134,22 +180,13 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                 s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
             }
         }
-        for (int g = 0; g < G; ++g) {
-            const long long q = s + g;
-            if (q >= s1) break;
-            const int k = a.all_traces ? (int)q : a.list[q];
-            const size_t row = (size_t)k * a.ns;
-            const float sc = (a.dtype == PSKTM_F32) ? 1.f : a.scale[k];
-            float2 *P = s_pairs + g * a.nsp;
-            for (int n = tid; n <= a.ns; n += NTH)
-                P[n] = make_pair(a.traces, a.dtype, row, n, a.ns, a.interp, sc);
-        }
+        fill_stage<NTH>(a, s, s1, s_pairs);
         __syncthreads();
         // ---- compute --------------------------------------------------------
         for (int g = 0; g < G; ++g) {
             const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
             if (!m || s + g >= s1) continue;
-            const float2 *P = s_pairs + g * a.nsp;
+            const float2 *P = s_pairs + g * a.nsp + kPairFront;
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
                 if (!((m >> b) & 1u)) continue;
@@ -206,6 +243,7 @@ constexpr int acc_regs(int tb, int tl, bool v3, bool kh)
     return tb * tl * ((kh ? 2 : 1) + (v3 ? 1 : 0));
 }
 constexpr int reg_budget(bool kh) { return kh ? 128 : 96; }
+
 
 constexpr int kTLSet[] = {2, 3, 4, 6, 8};
 

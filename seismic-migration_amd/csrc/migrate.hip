@@ -283,7 +283,7 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     FwdFn fn = fwd_kernel(sh.nth, tb, sh.tl, v3, kh);
     PSKTM_CHECK(fn, PSKTM_E_UNSUPPORTED, "no forward kernel for tile %d tl %d", tb, sh.tl);
 
-    const int nsp = (p->n_samples + 2) & ~1;
+    const int nsp = pair_stride(p->n_samples);
     const size_t slot = (size_t)nsp * sizeof(float2);
     int G = p->opt.traces_per_stage > 0 ? p->opt.traces_per_stage : 4;
     G = std::min(G, kGMax);

@@ -48,13 +48,13 @@ __global__ void key_to_float_kernel(const unsigned *__restrict__ key, int n, flo
 }
 
 __global__ void class_key_kernel(const float *__restrict__ hx, const float *__restrict__ hy, int n,
-                                 float inv_width, int n_classes, unsigned *__restrict__ key,
+                                 double width, int n_classes, unsigned *__restrict__ key,
                                  int *__restrict__ idx, int *__restrict__ counts)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double full = 2.0 * sqrt((double)hx[k] * hx[k] + (double)hy[k] * hy[k]);
-    const double c = floor(full * (double)inv_width);
+    const double c = floor(full / width);
     const int cls = (c >= 0.0 && c < (double)n_classes) ? (int)c : n_classes;
     key[k] = (unsigned)cls;
     idx[k] = k;
@@ -162,7 +162,7 @@ psktm_status psktm_offset_classes(const float *off_x, const float *off_y, int n,
         PSKTM_HIP(hipMalloc(&cnt.p, sizeof(int) * n_classes));
         PSKTM_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * n_classes, s));
         hipLaunchKernelGGL(class_key_kernel, dim3(blocks_for(n)), dim3(256), 0, s, off_x, off_y, n,
-                           1.0f / width, n_classes, (unsigned *)k0.p, (int *)i0.p, (int *)cnt.p);
+                           (double)width, n_classes, (unsigned *)k0.p, (int *)i0.p, (int *)cnt.p);
         int end_bit = 1;
         while ((1ll << end_bit) <= n_classes) ++end_bit;
         size_t bytes = 0;
@@ -220,13 +220,16 @@ psktm_status psktm_gather_pad(const float *in, int m, const int *rows, int n_row
 {
     psktm__clear_error();
     PSKTM_CHECK(m > 0 && n_rows >= 0 && sze > 0, PSKTM_E_INVAL, "bad sizes");
-    const int R = (n_rows + sze - 1) / sze * sze, C = (m + sze - 1) / sze * sze;
-    if ((size_t)R * C == 0) return PSKTM_OK;
+    const long long R = ((long long)n_rows + sze - 1) / sze * sze;
+    const long long C = ((long long)m + sze - 1) / sze * sze;
+    PSKTM_CHECK(R < (1ll << 31) && C < (1ll << 31) && R * C < (1ll << 40), PSKTM_E_INVAL,
+                "padded block too large");
+    if (R * C == 0) return PSKTM_OK;
     PSKTM_CHECK(in && out, PSKTM_E_INVAL, "NULL pointer");
     psktm_status st;
     if ((st = require_device()) != PSKTM_OK) return st;
     hipLaunchKernelGGL(gather_pad_kernel, dim3(blocks_for((size_t)R * C)), dim3(256), 0,
-                       as_stream(stream), in, m, rows, n_rows, R, C, out);
+                       as_stream(stream), in, m, rows, n_rows, (int)R, (int)C, out);
     return check_launch("gather_pad_kernel");
 }
 
@@ -235,7 +238,8 @@ psktm_status psktm_upsample_traces(const float *in, int n, int m, int N, float *
     psktm__clear_error();
     PSKTM_CHECK(n >= 0 && m >= 1 && N >= 0 && N <= 16, PSKTM_E_INVAL, "bad sizes or N");
     const long long mo = (long long)(m - 1) * (1ll << N) + 1;
-    PSKTM_CHECK(mo < (1ll << 31), PSKTM_E_INVAL, "upsampled trace too long");
+    PSKTM_CHECK(mo < (1ll << 31) && (long long)n * mo < (1ll << 40), PSKTM_E_INVAL,
+                "upsampled block too large");
     if (n == 0) return PSKTM_OK;
     PSKTM_CHECK(in && out, PSKTM_E_INVAL, "NULL pointer");
     psktm_status st;
