@@ -325,6 +325,65 @@ psktm_status psktm_gather_pad(const float *in, int m, const int *rows, int n_row
 psktm_status psktm_upsample_traces(const float *in, int n, int m, int N, float *out,
                                    void *stream);
 
+/* SEG-Y (rev 0) and SU ingest: the reference's MATLAB readers
+ * scripts/ktmGetInputMeta.m, scripts/ktmGetInputTrace.m and
+ * scripts/ktmGetTotalTraceNo.m, with the sample and header decode on the
+ * device. A SEG-Y file is a 3200-byte EBCDIC header, a 400-byte binary reel
+ * header and records of a 240-byte trace header plus n_samples samples; an
+ * SU file (name ending in ".su") is records only. Trace ranges are 0-based
+ * [first, first + count). */
+#define PSKTM_SEGY_HEADER_COLUMNS 101
+
+typedef struct {
+    long long data_offset;  /* byte offset of the first record: 3600 or 0 (SU) */
+    long long n_traces;     /* whole records in the file */
+    long long record_bytes; /* 240 + n_samples * bytes_per_sample */
+    int n_samples;          /* reel value 8 (SU: trace header bytes 115-116) */
+    int sample_interval_us; /* reel value 6 (SU: trace header bytes 117-118) */
+    int format;             /* 1 IBM float, 2 int32, 3 int16, 5 IEEE float */
+    int bytes_per_sample;   /* 2 for format 3, else 4 */
+    int big_endian;         /* byte order of trace headers and samples */
+    int ragged;             /* 1 if the file size is not a whole number of records */
+} psktm_segy_layout;
+
+/* ktmGetInputMeta(file, en, 1) (scripts/ktmGetInputMeta.m:30-56): the EBCDIC
+ * header through the script's table (unlisted codes -> '?'), 40 lines of 80
+ * characters each followed by '\n', then a NUL: 3241 bytes. */
+psktm_status psktm_segy_text_header(const char *path, char *out);
+/* ktmGetInputMeta(file, en, 2) (scripts/ktmGetInputMeta.m:58-76): job, line
+ * and reel numbers (4 bytes each) and 24 two-byte values, big-endian
+ * unsigned as the script composes them. */
+psktm_status psktm_segy_reel_header(const char *path, long long values[27]);
+/* Record layout and trace count (scripts/ktmGetTotalTraceNo.m:18-32). For
+ * SEG-Y the format and sample count come from the reel header; big_endian
+ * gives the byte order of the records (SEG-Y: 1). */
+psktm_status psktm_segy_open(const char *path, int big_endian, psktm_segy_layout *layout);
+/* ktmGetInputMeta(file, en, 3, range) (scripts/ktmGetInputMeta.m:78-153):
+ * count x 101 header columns (7 x int32, 4 x int16, 8 x int32, 2 x int16,
+ * 4 x int32 = source X/Y, receiver X/Y, then 76 x int16), host memory.
+ * saturate = 1 reproduces the script's int32()/int16() casts, which clamp
+ * negative values to 2147483647 / 32767; 0 decodes two's complement. */
+psktm_status psktm_segy_trace_headers(const psktm_segy_layout *layout, const char *path,
+                                      long long first, long long count, int saturate,
+                                      int *out);
+/* Decode `count` raw records already in device memory (record_bytes apart)
+ * into traces[count][n_samples] (FP32) and, if headers is not NULL,
+ * headers[count][101] (two's complement). Format 1 uses the reference's
+ * ktmIBM32toIEEE arithmetic when ibm_reference = 1 (bit-identical to it),
+ * the exact IBM value otherwise; formats 2 and 3 convert the integers; 5 is
+ * IEEE. Stream-ordered. */
+psktm_status psktm_segy_decode(const psktm_segy_layout *layout, const void *records,
+                               long long count, int ibm_reference, float *traces,
+                               int *headers, void *stream);
+/* File -> device: reads records [first, first + count) in chunks through
+ * pinned host buffers, copies them to the device and decodes them there
+ * (traces and optional headers as in psktm_segy_decode). Returns after the
+ * last chunk has been decoded on `stream`. The chunk size is 64 MiB unless
+ * the environment variable PSKTM_SEGY_CHUNK_BYTES sets another. */
+psktm_status psktm_segy_load(const psktm_segy_layout *layout, const char *path,
+                             long long first, long long count, int ibm_reference,
+                             float *traces, int *headers, void *stream);
+
 /* FP64 reductions for conjugate gradients, stream-ordered, no host sync:
  * result[0] = sum_i a[i]*b[i] accumulated in double. */
 psktm_status psktm_dot_f64(const float *a, const float *b, size_t n,

@@ -12,7 +12,8 @@ from .api import (Batch, Grid, Plan, Survey, demigrate, dot_f64, ibm32_to_ieee,
                   ibm32_to_ieee_device, ibm32_to_ieee_exact, init_three_axes,
                   init_two_axes, ktm_migration_gpu, load_ktm_config, migrate,
                   migrate_exact, quantize, rho_filter, offset_sort, offset_classes,
-                  select_offset, gather_pad, upsample_traces)
+                  select_offset, gather_pad, upsample_traces, segy_text_header,
+                  segy_reel_header, segy_open, segy_trace_headers, segy_decode, segy_load)
 
 __all__ = [
     "BF16", "F16", "F32", "INTERP_LINEAR", "INTERP_TRUNC", "MATH_FAST", "MATH_IEEE",
@@ -21,5 +22,6 @@ __all__ = [
     "ibm32_to_ieee_device", "ibm32_to_ieee_exact", "init_three_axes", "init_two_axes",
     "ktm_migration_gpu", "load_ktm_config", "migrate", "migrate_exact", "quantize",
     "rho_filter", "offset_sort", "offset_classes", "select_offset", "gather_pad",
-    "upsample_traces",
+    "upsample_traces", "segy_text_header", "segy_reel_header", "segy_open",
+    "segy_trace_headers", "segy_decode", "segy_load",
 ]

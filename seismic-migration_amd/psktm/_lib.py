@@ -64,6 +64,14 @@ class PsktmOptions(ctypes.Structure):
                 ("precise", _c_int)]
 
 
+class PsktmSegyLayout(ctypes.Structure):
+    _fields_ = [("data_offset", ctypes.c_longlong), ("n_traces", ctypes.c_longlong),
+                ("record_bytes", ctypes.c_longlong), ("n_samples", ctypes.c_int),
+                ("sample_interval_us", ctypes.c_int), ("format", ctypes.c_int),
+                ("bytes_per_sample", ctypes.c_int), ("big_endian", ctypes.c_int),
+                ("ragged", ctypes.c_int)]
+
+
 class PsktmGather(ctypes.Structure):
     _fields_ = [("grid", PsktmGrid), ("first_trace", _c_int), ("n_traces", _c_int)]
 
@@ -112,6 +120,15 @@ _SIGNATURES = {
     "psktm_select_offset": (_c_int, [_p, _p, _c_int, _c_float, _c_float, _p, _P(_c_int), _p]),
     "psktm_gather_pad": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _p, _p]),
     "psktm_upsample_traces": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p]),
+    "psktm_segy_text_header": (_c_int, [ctypes.c_char_p, ctypes.c_char_p]),
+    "psktm_segy_reel_header": (_c_int, [ctypes.c_char_p, _P(ctypes.c_longlong)]),
+    "psktm_segy_open": (_c_int, [ctypes.c_char_p, _c_int, _P(PsktmSegyLayout)]),
+    "psktm_segy_trace_headers": (_c_int, [_P(PsktmSegyLayout), ctypes.c_char_p,
+                                          ctypes.c_longlong, ctypes.c_longlong, _c_int, _p]),
+    "psktm_segy_decode": (_c_int, [_P(PsktmSegyLayout), _p, ctypes.c_longlong, _c_int, _p, _p,
+                                   _p]),
+    "psktm_segy_load": (_c_int, [_P(PsktmSegyLayout), ctypes.c_char_p, ctypes.c_longlong,
+                                 ctypes.c_longlong, _c_int, _p, _p, _p]),
     "psktm_dot_f64": (_c_int, [_p, _p, _c_size, _p, _p]),
     "psktm_axpy_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _c_float, _p]),
     "psktm_xpby_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _p]),

@@ -12,6 +12,7 @@ Two families, as in the C header:
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -397,6 +398,76 @@ def upsample_traces(traces, N, *, stream=None):
     check(_lib.load().psktm_upsample_traces(_p(_ptr(traces)), n, m, int(N), _p(_ptr(out)),
                                             _stream(stream)))
     return out
+
+
+# --------------------------------------------------------------------------
+# SEG-Y / SU ingest.
+# --------------------------------------------------------------------------
+
+SEGY_HEADER_COLUMNS = 101
+
+
+def segy_text_header(path):
+    """40 ASCII lines of the EBCDIC header (scripts/ktmGetInputMeta.m:30-56)."""
+    buf = ctypes.create_string_buffer(3241)
+    check(_lib.load().psktm_segy_text_header(os.fsencode(path), buf))
+    return buf.raw[:3240].decode("latin-1").split("\n")[:40]
+
+
+def segy_reel_header(path):
+    """The 27 binary reel values (scripts/ktmGetInputMeta.m:58-76)."""
+    import numpy as np
+    v = (ctypes.c_longlong * 27)()
+    check(_lib.load().psktm_segy_reel_header(os.fsencode(path), v))
+    return np.array(v[:], dtype=np.int64)
+
+
+def segy_open(path, big_endian=True):
+    """Record layout and trace count (scripts/ktmGetTotalTraceNo.m)."""
+    L = _lib.PsktmSegyLayout()
+    check(_lib.load().psktm_segy_open(os.fsencode(path), int(bool(big_endian)), ctypes.byref(L)))
+    return L
+
+
+def segy_trace_headers(layout, path, first, count, *, saturate=False):
+    """count x 101 header columns on the host (scripts/ktmGetInputMeta.m:78-153)."""
+    import numpy as np
+    out = np.zeros((int(count), SEGY_HEADER_COLUMNS), dtype=np.int32)
+    check(_lib.load().psktm_segy_trace_headers(ctypes.byref(layout), os.fsencode(path),
+                                               int(first), int(count), int(bool(saturate)),
+                                               _p(out.ctypes.data) if count else None))
+    return out
+
+
+def segy_decode(layout, records, count, *, ibm_reference=True, headers=True, stream=None):
+    """Decode raw records (a uint8 device tensor) into FP32 traces [count, ns]
+    and, optionally, int32 headers [count, 101]."""
+    torch = _torch()
+    dev = records.device
+    tr = torch.empty((int(count), layout.n_samples), dtype=torch.float32, device=dev)
+    hd = torch.empty((int(count), SEGY_HEADER_COLUMNS), dtype=torch.int32, device=dev) \
+        if headers else None
+    check(_lib.load().psktm_segy_decode(ctypes.byref(layout), _p(_ptr(records)), int(count),
+                                        int(bool(ibm_reference)), _p(_ptr(tr)),
+                                        _p(_ptr(hd)) if hd is not None else None,
+                                        _stream(stream)))
+    return (tr, hd) if headers else tr
+
+
+def segy_load(layout, path, first=0, count=None, *, ibm_reference=True, headers=True,
+              device="cuda", stream=None):
+    """Read records [first, first + count) from the file straight to device
+    tensors: FP32 traces [count, ns] and optional int32 headers [count, 101]."""
+    torch = _torch()
+    count = layout.n_traces - int(first) if count is None else int(count)
+    tr = torch.empty((count, layout.n_samples), dtype=torch.float32, device=device)
+    hd = torch.empty((count, SEGY_HEADER_COLUMNS), dtype=torch.int32, device=device) \
+        if headers else None
+    check(_lib.load().psktm_segy_load(ctypes.byref(layout), os.fsencode(path), int(first), count,
+                                      int(bool(ibm_reference)), _p(_ptr(tr)),
+                                      _p(_ptr(hd)) if hd is not None else None,
+                                      _stream(stream)))
+    return (tr, hd) if headers else tr
 
 
 # --------------------------------------------------------------------------

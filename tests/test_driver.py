@@ -13,25 +13,10 @@ import pytest
 
 import oracle
 import psktm
+from segy_files import ibm_encode
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "seismic-migration_amd", "psktm", "psktm_main")
-
-
-def ibm_encode(x):
-    """float -> IBM System/360 single (round to nearest fraction)."""
-    x = np.asarray(x, np.float64)
-    words = np.zeros(x.shape, np.uint32)
-    nz = x != 0
-    a = np.abs(x[nz])
-    e = np.floor(np.log2(a) / 4.0).astype(np.int64) + 1        # a = f * 16**e, f in [1/16, 1)
-    frac = np.round(a / 16.0 ** e * 2.0 ** 24).astype(np.int64)
-    over = frac >= 2 ** 24
-    frac[over] //= 16
-    e[over] += 1
-    sign = (x[nz] < 0).astype(np.int64)
-    words[nz] = ((sign << 31) | ((e + 64) << 24) | frac).astype(np.uint32)
-    return words
 
 
 def write_job(d, ntr=24, ns=200, grid=(8, 0.0, 20.0, 2, 0.0, 30.0, 150, 0.0, 0.004),
