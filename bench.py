@@ -89,6 +89,20 @@ def calibrate_copy(nbytes):
     print(json.dumps({"calibrate_copy_bytes": n * 4}), flush=True)
 
 
+def host_cpu():
+    """CPU model and logical CPUs of this host (recorded beside the CPU baseline)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -235,7 +249,7 @@ def main():
                         aperture=s.aperture, threads=cores)
         el = time.perf_counter() - t1
         cpu = {"value": pairs * grid.nt / el / 1e9, "unit": "G trace-sample updates/s",
-               "cores": cores, "kind": "port",
+               "cores": cores, "kind": "port", "host": host_cpu(),
                "sample": f"first {k1} traces of the workload, reference TRUNC binary32 loop "
                          f"with the same 1500 m aperture ({pairs * grid.nt:.3e} updates, "
                          f"{el:.1f} s)"}
@@ -552,7 +566,7 @@ def bench_line2d(args):
         "bit_identical_to_reference_loop": bit,
         "fast_linear_G_updates_per_s": updates / fast_s / 1e9,
         "cpu_baseline": {"value": updates / min(cpu) / 1e9, "unit": "G trace-sample updates/s",
-                         "cores": cores, "kind": "port",
+                         "cores": cores, "kind": "port", "host": host_cpu(),
                          "sample": "full 2-D line, best of 3, race-free restatement"},
         "cpu_racy_as_written_G_updates_per_s": updates / racy / 1e9,
     }
