@@ -48,12 +48,12 @@ __device__ __forceinline__ float decode16(unsigned bits, int dtype, float sc)
 // Stages the trace rows s .. s+G-1 of the tile list into the LDS pair table.
 // Work is split into (trace, group of 64 chunks) items, one item per wave at
 // a time, so the row address is wave-uniform and each row is read through a
-// buffer descriptor whose range ends at the row's last byte. Each lane loads
-// one 16-byte chunk (8 FP16/BF16 or 4 FP32 samples, from a 16-byte aligned
-// base) and the first sample of the next chunk, then writes one pair per
-// sample. A load whose start lies beyond the range returns zeros; a chunk
-// whose start lies inside it stays inside one 16-byte block that holds a byte
-// of the row, so no access leaves the pages of the trace buffer. Samples
+// buffer descriptor whose range ends at the dword holding the row's last byte.
+// Each lane loads one 16-byte chunk (8 FP16/BF16 or 4 FP32 samples, from a
+// 16-byte aligned base) and the first sample of the next chunk, then writes one
+// pair per sample. Dwords beyond the range read as zeros; a chunk whose start
+// lies inside it stays inside one 16-byte block that holds a byte of the row,
+// so no access leaves the pages of the trace buffer. Samples
 // outside [0, ns] land in the row's padding and are never read.
 template <int NTH>
 __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long long s1,
@@ -77,8 +77,10 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
         const int c = cg * 64 + lane;
         if (c >= n_chunks) continue;
         const float sc = a.dtype == PSKTM_F32 ? 1.f : a.scale[k];
+        // range rounded up to whole dwords: the bounds check is per dword, and
+        // a row of 16-bit samples may end in the middle of one
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(ra & ~(uintptr_t)15), (short)0, sb + a.ns * E, 0x00020000);
+            (void *)(ra & ~(uintptr_t)15), (short)0, (sb + a.ns * E + 3) & ~3, 0x00020000);
         const int n0 = (16 * c - sb) / E;           // first sample of the chunk
         float2 *P = s_pairs + g * a.nsp + kPairFront + n0;
         const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * c, 0, 0);
