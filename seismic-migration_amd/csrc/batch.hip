@@ -27,7 +27,7 @@ __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__res
                                                             int n_items,
                                                             unsigned *__restrict__ counter)
 {
-    extern __shared__ float2 s_pairs[];
+    extern __shared__ __align__(16) float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
     __shared__ unsigned s_mask[kGMax];
     __shared__ int s_item;

@@ -59,6 +59,9 @@ template <int NTH>
 __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long long s1,
                                            float2 *__restrict__ s_pairs)
 {
+    // the pair table is (fl(h * sc), fl(d[n+1] - d[n])) exactly, whatever the
+    // surrounding code: no fused multiply-subtract
+#pragma clang fp contract(off)
     constexpr int W = NTH / 64;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -93,14 +96,32 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
             return decode16(w, a.dtype, sc);
         };
         float cur = sample(0);
+        if ((n0 & 1) == 0) {
+            // entry n0 starts a 16-byte boundary: two pairs per ds_write_b128
+            // (half as many bank-conflicted writes as one pair per write)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if (i >= per) break;
-            const float nxt = sample(i + 1);
-            const int n = n0 + i;
-            const bool ok = lin ? (n >= 0 && n <= a.ns - 2) : (n >= 1 && n <= a.ns - 1);
-            P[i] = ok ? make_float2(cur, lin ? nxt - cur : 0.f) : make_float2(0.f, 0.f);
-            cur = nxt;
+            for (int i = 0; i < 8; i += 2) {
+                if (i >= per) break;
+                const float n1 = sample(i + 1), n2 = sample(i + 2);
+                const int n = n0 + i;
+                const bool ok0 = lin ? (n >= 0 && n <= a.ns - 2) : (n >= 1 && n <= a.ns - 1);
+                const bool ok1 = lin ? (n + 1 >= 0 && n + 1 <= a.ns - 2)
+                                     : (n + 1 >= 1 && n + 1 <= a.ns - 1);
+                const float2 p0 = ok0 ? make_float2(cur, lin ? n1 - cur : 0.f) : make_float2(0.f, 0.f);
+                const float2 p1 = ok1 ? make_float2(n1, lin ? n2 - n1 : 0.f) : make_float2(0.f, 0.f);
+                reinterpret_cast<float4 *>(P)[i >> 1] = make_float4(p0.x, p0.y, p1.x, p1.y);
+                cur = n2;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i >= per) break;
+                const float nxt = sample(i + 1);
+                const int n = n0 + i;
+                const bool ok = lin ? (n >= 0 && n <= a.ns - 2) : (n >= 1 && n <= a.ns - 1);
+                P[i] = ok ? make_float2(cur, lin ? nxt - cur : 0.f) : make_float2(0.f, 0.f);
+                cur = nxt;
+            }
         }
     }
 }

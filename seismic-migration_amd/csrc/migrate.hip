@@ -58,7 +58,7 @@ struct AdjArgs {
 template <int NTH, int TB, int TL, bool V3, bool KH>
 __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 {
-    extern __shared__ float2 s_pairs[];
+    extern __shared__ __align__(16) float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
     __shared__ unsigned s_mask[kGMax];
     fwd_tile<NTH, TB, TL, V3, KH>(a, a.tile_order[blockIdx.x / a.n_chunks],
