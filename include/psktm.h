@@ -188,6 +188,12 @@ typedef struct psktm_options {
                                  the forward, FP64 accumulation in the
                                  adjoint; for least squares and adjoint
                                  tests. 0: plain FP32 accumulation. */
+    int   splits;             /* forward: workgroups sharing one tile's trace
+                                 list, each summing a contiguous part into a
+                                 partial image that a second kernel adds in
+                                 fixed order. 0 = library choice (more than
+                                 one only when the image has too few tiles to
+                                 fill the GPU), 1 = one workgroup per tile. */
 } psktm_options;
 
 /* Last error text of the calling thread ("" if none). */

@@ -40,8 +40,8 @@ __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__res
         const int g = __builtin_amdgcn_readfirstlane(items[it].x);
         const int tc = __builtin_amdgcn_readfirstlane(items[it].y);
         const FwdArgs a = descs[g];
-        fwd_tile<NTH, TB, TL, false, KH>(a, tc / a.n_chunks, tc % a.n_chunks, s_pairs, s_ab,
-                                         s_mask);
+        fwd_tile<NTH, TB, TL, false, KH>(a, tc / a.n_chunks, tc % a.n_chunks, 0, s_pairs,
+                                         s_ab, s_mask);
     }
 }
 
@@ -330,6 +330,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
         a.G = b->G;
         a.nsp = b->nsp;
         a.accumulate = 0;
+        a.n_split = 1;
         a.traces = (const char *)tr->data + (size_t)b->gathers[g].first_trace * b->ns * esz;
         a.scale = tr->dtype == PSKTM_F32 ? nullptr : tr->scale + b->gathers[g].first_trace;
         a.slow = b->slow;

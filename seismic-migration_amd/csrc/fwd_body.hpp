@@ -19,6 +19,9 @@ struct FwdArgs {
     int tile_x, tile_y, tiles_y, n_chunks;
     float nsf, r2, wscale;
     int use_ap, all_traces, ntr, interp, dtype, G, nsp, accumulate;
+    int n_split;                  // workgroups per (tile, tau pass); > 1 writes partials
+    float *partial;               // [n_split][image] when n_split > 1
+    long long image_elems;
     const float4 *geo;
     const int *list;
     const long long *tile_off;
@@ -144,6 +147,7 @@ __device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, fl
 // persistent batch kernel; the caller provides the LDS arrays.
 template <int NTH, int TB, int TL, bool V3, bool KH>
 __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const int chunk,
+                                         const int split,
                                          float2 *__restrict__ s_pairs,
                                          float2 (*__restrict__ s_ab)[TB],
                                          unsigned *__restrict__ s_mask)
@@ -179,8 +183,11 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
         }
     }
 
-    const long long s0 = a.all_traces ? 0 : a.tile_off[T];
-    const long long s1 = a.all_traces ? a.ntr : a.tile_off[T + 1];
+    // this workgroup's part of the tile's trace list (split `split` of n_split)
+    const long long l0 = a.all_traces ? 0 : a.tile_off[T];
+    const long long l1 = a.all_traces ? a.ntr : a.tile_off[T + 1];
+    const long long s0 = l0 + (l1 - l0) * split / a.n_split;
+    const long long s1 = l0 + (l1 - l0) * (split + 1) / a.n_split;
     const int G = a.G;
     for (long long s = s0; s < s1; s += G) {
         // ---- fill: distance terms + aperture bits, then trace pairs -------
@@ -253,7 +260,10 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
             const float sum = KH ? acc[b][j] - cmp[KH ? b : 0][KH ? j : 0] : acc[b][j];
             const float v = sum * w;
-            a.image[idx] = a.accumulate ? a.image[idx] + v : v;
+            if (a.n_split > 1)
+                a.partial[(size_t)split * a.image_elems + idx] = v;
+            else
+                a.image[idx] = a.accumulate ? a.image[idx] + v : v;
         }
     }
 }

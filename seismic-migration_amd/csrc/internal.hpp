@@ -118,6 +118,8 @@ struct psktm_plan {
     size_t slow_cap = 0;
     float *tsq = nullptr;
     size_t tsq_cap = 0;
+    float *partial = nullptr;       // split-K partial images [splits][image]
+    size_t partial_cap = 0;
 
     size_t bytes = 0;
 };

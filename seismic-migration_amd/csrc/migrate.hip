@@ -61,8 +61,11 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
     extern __shared__ __align__(16) float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
     __shared__ unsigned s_mask[kGMax];
-    fwd_tile<NTH, TB, TL, V3, KH>(a, a.tile_order[blockIdx.x / a.n_chunks],
-                                  blockIdx.x % a.n_chunks, s_pairs, s_ab, s_mask);
+    // block -> (tile in work order, tau pass, split); splits of a tile adjacent
+    const int split = blockIdx.x % a.n_split;
+    const int rest = blockIdx.x / a.n_split;
+    fwd_tile<NTH, TB, TL, V3, KH>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks, split,
+                                  s_pairs, s_ab, s_mask);
 }
 
 template <int NTH, int TB, int TL, bool V3, typename Acc>
@@ -191,6 +194,17 @@ __global__ void zero_kernel(float *p, size_t n)
     if (i < n) p[i] = 0.f;
 }
 
+// image[i] (+)= sum over splits of partial[s][i], in split order.
+__global__ void reduce_partials_kernel(const float *__restrict__ partial, int n_split,
+                                       long long n, int accumulate, float *__restrict__ image)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = partial[i];
+    for (int s = 1; s < n_split; ++s) t += partial[(size_t)s * n + i];
+    image[i] = accumulate ? image[i] + t : t;
+}
+
 // ---- kernel selection ------------------------------------------------------
 
 using FwdFn = void (*)(FwdArgs);
@@ -302,9 +316,58 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     a.traces = tr.data; a.scale = tr.scale; a.slow = p->slow; a.tsq = p->tsq; a.image = image;
     const size_t lds = (size_t)G * slot;
     { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }
-    const unsigned blocks = (unsigned)((size_t)p->n_tiles * sh.chunks);
-    hipLaunchKernelGGL(fn, dim3(blocks), dim3(sh.nth), lds, st, a);
-    return check_launch("migrate_fwd_kernel");
+
+    // Split-K: when the image has too few (tile, tau pass) work units to fill
+    // the GPU twice over, several workgroups share each tile's trace list and
+    // write partial images that reduce_partials_kernel adds in split order.
+    const long long base = (long long)p->n_tiles * sh.chunks;
+    int n_split = p->opt.splits;
+    if (n_split == 0) {
+        int cus = 0, per_cu = 0;
+        PSKTM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+        PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn,
+                                                               sh.nth, lds));
+        const long long target = 2ll * cus * std::max(per_cu, 1);
+        const double per_tile = p->all_traces ? (double)p->n_traces
+                                              : (double)p->list_len / std::max(p->n_tiles, 1);
+        n_split = 1;
+        if (base > 0 && base < target) {
+            const long long want = (target + base - 1) / base;
+            const long long by_len = (long long)(per_tile / (4.0 * G));   // >= 4 stages each
+            n_split = (int)std::max(1ll, std::min({want, by_len, 64ll}));
+        }
+    }
+    a.n_split = n_split;
+    a.image_elems = (long long)g.nx * g.ny * g.nt;
+    if (n_split > 1) {
+        const size_t need = (size_t)n_split * (size_t)a.image_elems;
+        if (p->partial_cap < need) {
+            if (p->partial) {
+                PSKTM_HIP(hipStreamSynchronize(st));
+                (void)hipFree(p->partial);
+                p->bytes -= p->partial_cap * sizeof(float);
+                p->partial = nullptr;
+                p->partial_cap = 0;
+            }
+            PSKTM_HIP(hipMalloc(&p->partial, need * sizeof(float)));
+            p->partial_cap = need;
+            p->bytes += need * sizeof(float);
+        }
+        a.partial = p->partial;
+    }
+    const long long blocks = base * n_split;
+    PSKTM_CHECK(blocks < (1ll << 31), PSKTM_E_UNSUPPORTED, "too many workgroups (%lld)", blocks);
+    if (blocks > 0) {
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(sh.nth), lds, st, a);
+        if ((s = check_launch("migrate_fwd_kernel")) != PSKTM_OK) return s;
+    }
+    if (n_split > 1) {
+        const long long n = a.image_elems;
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           st, (const float *)p->partial, n_split, n, accumulate, image);
+        return check_launch("reduce_partials_kernel");
+    }
+    return PSKTM_OK;
 }
 
 psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_velocity &vel,

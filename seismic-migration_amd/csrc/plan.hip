@@ -225,7 +225,7 @@ static void plan_free(psktm_plan *p)
 {
     if (!p) return;
     void *bufs[] = {p->geo, p->sorted_ids, p->cell_start, p->tile_off, p->list,
-                    p->tile_order, p->adj_items, p->slow, p->tsq};
+                    p->tile_order, p->adj_items, p->slow, p->tsq, p->partial};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -431,6 +431,8 @@ psktm_status psktm_plan_create(psktm_plan **plan, const psktm_grid *grid,
     PSKTM_CHECK(opt->math == PSKTM_MATH_FAST, PSKTM_E_UNSUPPORTED,
                 "plans run the FAST form; use psktm_migrate_exact for IEEE arithmetic");
     PSKTM_CHECK(std::isfinite(opt->aperture), PSKTM_E_INVAL, "aperture must be finite");
+    PSKTM_CHECK(opt->splits >= 0 && opt->splits <= 1024, PSKTM_E_INVAL,
+                "splits must be in [0, 1024]");
     PSKTM_CHECK((grid->nx == 1 || grid->dx > 0.0f) && (grid->ny == 1 || grid->dy > 0.0f),
                 PSKTM_E_INVAL, "bin spacings must be positive");
     int tx = opt->tile_x, ty = opt->tile_y;

@@ -128,11 +128,12 @@ class Plan:
 
     def __init__(self, grid: Grid, survey: Survey, *, interp=INTERP_LINEAR, math=MATH_FAST,
                  aperture=0.0, tile_x=0, tile_y=0, traces_per_stage=0, precise=False,
-                 stream=None):
+                 splits=0, stream=None):
         L = _lib.load()
         self.grid, self.survey = grid, survey
         self.opt = PsktmOptions(int(interp), int(math), float(aperture), int(tile_x),
-                                int(tile_y), int(traces_per_stage), int(bool(precise)))
+                                int(tile_y), int(traces_per_stage), int(bool(precise)),
+                                int(splits))
         self._h = _p()
         g, s = grid.c(), survey.c()
         check(L.psktm_plan_create(ctypes.byref(self._h), ctypes.byref(g), ctypes.byref(s),
@@ -181,7 +182,7 @@ class Batch:
         for q, (g, f, n) in enumerate(self.gathers):
             arr[q] = PsktmGather(g.c(), f, n)
         self.opt = PsktmOptions(int(interp), int(MATH_FAST), float(aperture), int(tile_x),
-                                int(tile_y), int(traces_per_stage), int(bool(precise)))
+                                int(tile_y), int(traces_per_stage), int(bool(precise)), 1)
         self._h = _p()
         s = survey.c()
         check(L.psktm_batch_create(ctypes.byref(self._h), arr, len(self.gathers),

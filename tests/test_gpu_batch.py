@@ -37,7 +37,8 @@ def test_batch_equals_per_gather_plans(cuda):
         s = psktm.Survey(gd.mid_x[sl].contiguous(), gd.mid_y[sl].contiguous(),
                          gd.off_x[sl].contiguous(), gd.off_y[sl].contiguous(),
                          gd.traces.shape[1], gd.dt)
-        plan = psktm.Plan(psktm.Grid(*grid), s, aperture=gd.aperture)
+        # one workgroup per tile: the batch kernel's summation order
+        plan = psktm.Plan(psktm.Grid(*grid), s, aperture=gd.aperture, splits=1)
         total += plan.updates()
         one = psktm.migrate(plan, q[sl].contiguous(), gd.vel, scale=sc[sl].contiguous())
         assert torch.equal(batch.image(imgs, g), one), g

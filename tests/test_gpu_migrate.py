@@ -192,3 +192,27 @@ def test_odd_sample_counts_and_unaligned_rows(cuda, ns, offset):
         errs[str(dtype)] = rel_l2(img, ref)
     print("rel-L2", errs)
     assert max(errs.values()) <= 1e-4, errs
+
+
+def test_split_tile_lists_agree(cuda):
+    """Several workgroups per tile (partial images added in fixed order) give
+    the one-workgroup image up to summation rounding, deterministically, also
+    in accumulate mode and with more splits than a tile has traces."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=13, ntr=2500, nx=8, ny=6, nt=300)
+    ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+    imgs = {}
+    for splits in (1, 3, 7, 0, 1024):
+        plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, splits=splits)
+        a = psktm.migrate(plan, t(tr), t(vel))
+        b = psktm.migrate(plan, t(tr), t(vel))
+        assert torch.equal(a, b), splits                      # deterministic
+        c = a.clone()
+        psktm.migrate(plan, t(tr), t(vel), image=c, accumulate=True)
+        assert torch.equal(c, a + a), splits
+        imgs[splits] = a.cpu().numpy()
+        assert rel_l2(imgs[splits], ref) <= 1e-4, splits
+    for k, v in imgs.items():
+        assert rel_l2(v, imgs[1]) <= 2e-6, k
+    with pytest.raises(psktm.PsktmError):
+        gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, splits=-1)
