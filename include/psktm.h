@@ -185,9 +185,9 @@ typedef struct psktm_options {
     int   tile_x, tile_y;     /* bins per tile; 0 = library choice */
     int   traces_per_stage;   /* traces staged in LDS at once; 0 = choice */
     int   precise;            /* 1: compensated (Kahan) FP32 accumulation in
-                                 the forward, FP64 accumulation in the
-                                 adjoint; for least squares and adjoint
-                                 tests. 0: plain FP32 accumulation. */
+                                 the forward, for least squares and adjoint
+                                 tests. 0: plain FP32 accumulation. The
+                                 adjoint accumulates in FP64 either way. */
     int   splits;             /* forward: workgroups sharing one tile's trace
                                  list, each summing a contiguous part into a
                                  partial image that a second kernel adds in

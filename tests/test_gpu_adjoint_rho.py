@@ -17,14 +17,15 @@ def dot64(a, b):
     return float(np.sum(np.asarray(a, np.float64) * np.asarray(b, np.float64)))
 
 
-@pytest.mark.parametrize("precise,tol", [(True, 1e-6), (False, 2e-5)])
+@pytest.mark.parametrize("precise,tol", [(True, 1e-6), (False, 2e-6)])
 @pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
 def test_dot_product_test_3d(cuda, interp, precise, tol):
     """<F d, m> = <d, F' m> with zero-mean random d, m and FP64 inner products.
 
-    Precise mode (Kahan forward, FP64 adjoint accumulation) meets 1e-6. Plain
-    FP32 accumulation of K ~ 2000 random-sign terms per image sample carries a
-    relative error ~ eps*sqrt(K/2) ~ 2e-6 per sample, so its bound is 2e-5.
+    The adjoint accumulates in FP64 in both modes. Precise mode (Kahan
+    forward) meets 1e-6. The plain forward's FP32 sum of K ~ 2000 random-sign
+    terms per image sample has a relative error ~ eps*sqrt(K/2) ~ 2e-6 per
+    sample; summed over many samples the test sees ~4e-7, bounded here by 2e-6.
     """
     import torch
     grid, mid, off, tr, vel, dt, ap = small_3d(seed=11, ntr=2000, nx=10, ny=9)
