@@ -394,7 +394,7 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
     // traces per adjoint work item: up to 4 (measured faster than 8: two
     // workgroups per CU), as many as FP64 accumulators fit in 150 KB of LDS
     p->adj_batch = (int)std::max(1ll, std::min(4ll, (150ll * 1024) /
-                                                        (8ll * (s.n_samples + 2))));
+                                                        (8ll * adj_row(s.n_samples))));
     std::vector<int> items;
     for (int c = 0; c < p->n_cells; ++c)
         for (int q = h_start[c]; q < h_start[c + 1]; q += p->adj_batch) {
