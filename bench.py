@@ -206,9 +206,16 @@ def main():
         sel = torch.nonzero(ex * ex + ey * ey <= (s.aperture * 1.001) ** 2).squeeze(1)
         host = lambda t: t.index_select(0, sel).cpu().numpy()
         t1 = time.time()
-        ref = oracle.pstm_f64(oracle.Grid(*s.grid), host(s.mid_x), host(s.mid_y), host(s.off_x),
-                              host(s.off_y), host(f32_traces), s.vel.cpu().numpy(), dt,
-                              vel3d=True, aperture=s.aperture, interp=oracle.LINEAR, bins=bins)
+        hmx, hmy, hox, hoy = host(s.mid_x), host(s.mid_y), host(s.off_x), host(s.off_y)
+        htr, hvel = host(f32_traces), s.vel.cpu().numpy()
+        ref = np.zeros((nb, nb, grid.nt))
+        rows = 4                      # a few bin rows per call, with progress lines
+        for r0 in range(0, nb, rows):
+            r1 = min(nb, r0 + rows)
+            ref[r0:r1] = oracle.pstm_f64(oracle.Grid(*s.grid), hmx, hmy, hox, hoy, htr, hvel, dt,
+                                         vel3d=True, aperture=s.aperture, interp=oracle.LINEAR,
+                                         bins=(c0x + r0, c0x + r1, c0y, c0y + nb))
+            log(f"FP64 oracle: {r1}/{nb} bin rows ({time.time() - t1:.0f} s)")
         log(f"FP64 oracle on {nb}x{nb} bins x {grid.nt} ({len(sel)} traces) in {time.time() - t1:.1f} s")
         plan = psktm.Plan(grid, survey, aperture=s.aperture)
         for form in ("f32", "f16", "bf16"):
@@ -370,28 +377,36 @@ def bench_batched(args):
         batch.migrate(q, gd.vel, images, scale=sc, use_graph=True)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t2) / args.steps
-    # the same gathers launched one plan at a time
-    plans = []
-    for g, f, c in gathers:
-        sl = slice(f, f + c)
-        s = psktm.Survey(gd.mid_x[sl].contiguous(), gd.mid_y[sl].contiguous(),
-                         gd.off_x[sl].contiguous(), gd.off_y[sl].contiguous(), ns, gd.dt)
-        plans.append((psktm.Plan(g, s, aperture=gd.aperture, tile_x=choice[0], tile_y=choice[1],
-                                 traces_per_stage=choice[2]), sl))
-    seq_imgs = [torch.empty(g.shape, dtype=torch.float32, device=dev) for g, _, _ in gathers]
+    # the same gathers launched one plan at a time: with one workgroup per
+    # tile (the batch kernel's summation order, so images must be identical)
+    # and with the library's choice (split tile lists for small gathers)
+    def sequential_run(splits):
+        plans = []
+        for g, f, c in gathers:
+            sl = slice(f, f + c)
+            s = psktm.Survey(gd.mid_x[sl].contiguous(), gd.mid_y[sl].contiguous(),
+                             gd.off_x[sl].contiguous(), gd.off_y[sl].contiguous(), ns, gd.dt)
+            plans.append((psktm.Plan(g, s, aperture=gd.aperture, tile_x=choice[0],
+                                     tile_y=choice[1], traces_per_stage=choice[2],
+                                     splits=splits), sl))
+        imgs = [torch.empty(g.shape, dtype=torch.float32, device=dev) for g, _, _ in gathers]
 
-    def sequential():
-        for (p, sl), img in zip(plans, seq_imgs):
-            psktm.migrate(p, q[sl], gd.vel, img, scale=sc[sl])
+        def sequential():
+            for (p, sl), img in zip(plans, imgs):
+                psktm.migrate(p, q[sl], gd.vel, img, scale=sc[sl])
 
-    sequential()
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    for _ in range(args.steps):
         sequential()
-    torch.cuda.synchronize()
-    seq = (time.perf_counter() - t3) / args.steps
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(args.steps):
+            sequential()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t3) / args.steps, imgs
+
+    seq, seq_imgs = sequential_run(1)
     same = all(torch.equal(batch.image(images, k), seq_imgs[k]) for k in range(len(gathers)))
+    del seq_imgs
+    seq_split, _ = sequential_run(0)
     line = {
         "metric": "G trace-sample updates/s (batched offset-class gathers, one persistent launch)",
         "value": updates / wall / 1e9, "unit": "G trace-sample updates/s", "n_gpus": 1,
@@ -405,6 +420,8 @@ def bench_batched(args):
                    "tile": list(choice), "graph": True},
         "sequential_ms": seq * 1e3, "persistent_over_sequential": seq / wall,
         "images_equal_sequential": bool(same),
+        "sequential_split_ms": seq_split * 1e3,
+        "persistent_over_sequential_split": seq_split / wall,
         "autotuner": tuner.cache.get(key),
     }
     print(json.dumps(line), flush=True)
