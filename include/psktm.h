@@ -194,6 +194,16 @@ typedef struct psktm_options {
                                  fixed order. 0 = library choice (more than
                                  one only when the image has too few tiles to
                                  fill the GPU), 1 = one workgroup per tile. */
+    int   row_pad;            /* forward: zero entries appended to every trace
+                                 row staged in LDS. A tile whose traveltimes
+                                 are bounded (from the plan's geometry and the
+                                 tile's slowness) below the padded row's end
+                                 reads the row without clamping the sample
+                                 index; images are bit-identical either way.
+                                 0 = library choice (up to half a row, as
+                                 long as the kernel's occupancy is kept),
+                                 -1 = no padding (the index is always
+                                 clamped), > 0 = that many entries. */
 } psktm_options;
 
 /* Last error text of the calling thread ("" if none). */

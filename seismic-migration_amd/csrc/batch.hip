@@ -105,7 +105,7 @@ struct psktm_batch {
     float dt = 0.f, tau0 = 0.f, dtau = 0.f;
     psktm::FwdShape shape{};
     psktm::BatchFn fn = nullptr;
-    int G = 4, nsp = 0;
+    int G = 4, nsp = 0, lim = 0;
     size_t lds = 0;
     int blocks = 0;
     int2 *d_items = nullptr;
@@ -217,8 +217,7 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
         psktm__set_error("no batch kernel for tile %d", b->tb);
         return PSKTM_E_UNSUPPORTED;
     }
-    b->nsp = pair_stride(b->ns);
-    const size_t slot = (size_t)b->nsp * sizeof(float2);
+    const size_t slot = (size_t)pair_stride(b->ns) * sizeof(float2);
     int G = opt->traces_per_stage > 0 ? opt->traces_per_stage : 4;
     G = std::min(G, kGMax);
     while (G > 1 && (size_t)G * slot > 96 * 1024) --G;
@@ -227,8 +226,21 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
         psktm__set_error("traces of %d samples do not fit in LDS", b->ns);
         return PSKTM_E_UNSUPPORTED;
     }
+    int pad = 0;
+    if ((st = choose_row_pad(opt->row_pad, b->ns, G, (const void *)b->fn, b->shape.nth, &pad)) !=
+        PSKTM_OK) {
+        batch_free(b);
+        return st;
+    }
+    b->nsp = pair_stride(b->ns, pad);
+    b->lim = opt->row_pad < 0 ? 0 : b->nsp - kPairFront;
+    if ((size_t)G * b->nsp * sizeof(float2) > 150 * 1024) {
+        batch_free(b);
+        psktm__set_error("row_pad %d does not fit in LDS", pad);
+        return PSKTM_E_UNSUPPORTED;
+    }
     b->G = G;
-    b->lds = (size_t)G * slot;
+    b->lds = (size_t)G * b->nsp * sizeof(float2);
     if ((st = ensure_dynamic_lds((const void *)b->fn, b->lds)) != PSKTM_OK) {
         batch_free(b);
         return st;
@@ -329,6 +341,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
         a.dtype = tr->dtype;
         a.G = b->G;
         a.nsp = b->nsp;
+        a.lim = b->lim;
         a.accumulate = 0;
         a.n_split = 1;
         a.traces = (const char *)tr->data + (size_t)b->gathers[g].first_trace * b->ns * esz;

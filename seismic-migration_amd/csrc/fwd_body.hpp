@@ -7,6 +7,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace psktm {
 
@@ -22,6 +23,8 @@ struct FwdArgs {
     int n_split;                  // workgroups per (tile, tau pass); > 1 writes partials
     float *partial;               // [n_split][image] when n_split > 1
     long long image_elems;
+    int lim;                      // pair-table entries from n = 0 in every staged row
+    const float *reach;           // plan's bound on |s - x| and |r - x| (device scalar)
     const float4 *geo;
     const int *list;
     const long long *tile_off;
@@ -38,9 +41,13 @@ struct FwdArgs {
 // LINEAR (valid n in [0, ns-2]) or (d[n], 0) for TRUNC (valid n in [1, ns-1]),
 // and (0, 0) elsewhere; d = stored sample x per-trace scale. A row keeps
 // kPairFront entries in front of n = 0 and enough behind n = ns that the
-// 16-byte chunk writes of the fill never leave the row.
+// 16-byte chunk writes of the fill never leave the row (the fill writes
+// entries -7 .. ns+7), plus `pad` zero entries that no fill overwrites.
 constexpr int kPairFront = 8;
-__host__ __device__ inline int pair_stride(int ns) { return (ns + 2 * kPairFront + 1) & ~1; }
+__host__ __device__ inline int pair_stride(int ns, int pad = 0)
+{
+    return (ns + pad + 2 * kPairFront + 1) & ~1;
+}
 
 __device__ __forceinline__ float decode16(unsigned bits, int dtype, float sc)
 {
@@ -183,12 +190,42 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
         }
     }
 
+    // The pair table is cleared once per tile: entries past what the fill
+    // writes (-7 .. ns+7 of each row) then read as zeros for the whole tile.
+    for (int q = tid; q < a.G * a.nsp; q += NTH) s_pairs[q] = make_float2(0.f, 0.f);
+    // Traveltime bound of the tile: every (trace, in-aperture bin) pair has
+    // A, B <= reach^2, so x <= 2 sqrt(tau'^2 + reach^2 s') at each tau. When
+    // that stays inside the padded rows (with a margin far above the FP32
+    // rounding of A, B and v_sqrt_f32), the index needs no clamp: samples
+    // at and past ns read zero pairs either way, so the image is the same.
+    bool unclamped = false;
+    {
+        float xb = 0.f;
+        if (a.lim > a.ns) {
+            const float r = *a.reach;
+            const float r2 = r * r;
+#pragma unroll
+            for (int j = 0; j < TL; ++j) {
+                float sm = sv[0][j];
+                if (V3) {
+#pragma unroll
+                    for (int b = 1; b < TB; ++b) sm = fmaxf(sm, sv[V3 ? b : 0][j]);
+                }
+                xb = fmaxf(xb, sqrtf(fmaf(r2, sm, tq[j])));
+            }
+        }
+        const bool ok = a.lim > a.ns && 2.f * xb * 1.0001f + 2.f < (float)a.lim;
+        unclamped = !__syncthreads_or(!ok);
+    }
+
     // this workgroup's part of the tile's trace list (split `split` of n_split)
     const long long l0 = a.all_traces ? 0 : a.tile_off[T];
     const long long l1 = a.all_traces ? a.ntr : a.tile_off[T + 1];
     const long long s0 = l0 + (l1 - l0) * split / a.n_split;
     const long long s1 = l0 + (l1 - l0) * (split + 1) / a.n_split;
     const int G = a.G;
+    auto stages = [&](auto clamp_tag) {
+    constexpr bool CL = decltype(clamp_tag)::value;
     for (long long s = s0; s < s1; s += G) {
         // ---- fill: distance terms + aperture bits, then trace pairs -------
         if (tid < G * TB) {
@@ -227,7 +264,7 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                     const float ua = fmaf(ab.x, sl, tq[j]);
                     const float ub = fmaf(ab.y, sl, tq[j]);
                     float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
-                    x = fminf(x, a.nsf);
+                    if (CL) x = fminf(x, a.nsf);
                     const int n = (int)x;
                     const float f = __builtin_amdgcn_fractf(x);
                     const float2 pr = P[n];
@@ -246,6 +283,11 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
         }
         __syncthreads();
     }
+    };
+    if (unclamped)
+        stages(std::false_type{});
+    else
+        stages(std::true_type{});
     // ---- epilogue: weight tau*sqrt(dt), one write per sample ---------------
 #pragma unroll
     for (int b = 0; b < TB; ++b) {
@@ -313,6 +355,30 @@ inline void fwd_args_from_plan(const psktm_plan *p, FwdArgs &a)
     a.ntr = p->n_traces;
     a.interp = p->opt.interp;
     a.geo = p->geo; a.list = p->list; a.tile_off = p->tile_off; a.tile_order = p->tile_order;
+    a.reach = p->reach;
+}
+
+// Zero entries appended to every staged row (psktm_options.row_pad): -1 none,
+// > 0 that many, 0 the largest of ns/4, ns/8, ... (at least 16) whose LDS
+// footprint keeps the kernel's resident workgroups per CU and stays within
+// 64 KiB (or the unpadded footprint, if that is larger).
+inline psktm_status choose_row_pad(int opt_pad, int ns, int G, const void *fn, int nth, int *pad)
+{
+    *pad = 0;
+    if (opt_pad < 0) return PSKTM_OK;
+    if (opt_pad > 0) { *pad = opt_pad; return PSKTM_OK; }
+    const size_t base_lds = (size_t)G * pair_stride(ns) * sizeof(float2);
+    const size_t cap = std::max<size_t>(base_lds, 64 * 1024);
+    int base = 0;
+    PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, nth, base_lds));
+    for (int cand = ns / 4; cand >= 16; cand /= 2) {
+        const size_t lds = (size_t)G * pair_stride(ns, cand) * sizeof(float2);
+        if (lds > cap) continue;
+        int occ = 0;
+        PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nth, lds));
+        if (occ >= base) { *pad = cand; break; }
+    }
+    return PSKTM_OK;
 }
 
 } // namespace psktm

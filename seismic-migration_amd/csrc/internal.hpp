@@ -111,6 +111,9 @@ struct psktm_plan {
     long long *tile_off = nullptr;  // [n_tiles + 1] offsets into list
     int *list = nullptr;            // per-tile trace ids, tile after tile
     int *tile_order = nullptr;      // tiles by decreasing work
+    float *reach = nullptr;         // [1] bound on |source - bin| and
+                                    // |receiver - bin| over the plan's
+                                    // (trace, in-aperture bin) pairs
     int *adj_items = nullptr;       // [n_adj_items * 2] (cell, first sorted pos)
     int n_adj_items = 0;
     int adj_batch = 0;              // traces per adjoint work item

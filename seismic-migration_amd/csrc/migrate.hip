@@ -332,13 +332,18 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     FwdFn fn = fwd_kernel(sh.nth, tb, sh.tl, v3, kh);
     PSKTM_CHECK(fn, PSKTM_E_UNSUPPORTED, "no forward kernel for tile %d tl %d", tb, sh.tl);
 
-    const int nsp = pair_stride(p->n_samples);
-    const size_t slot = (size_t)nsp * sizeof(float2);
+    const size_t slot = (size_t)pair_stride(p->n_samples) * sizeof(float2);
     int G = p->opt.traces_per_stage > 0 ? p->opt.traces_per_stage : 4;
     G = std::min(G, kGMax);
     while (G > 1 && (size_t)G * slot > 96 * 1024) --G;
     PSKTM_CHECK((size_t)G * slot <= 150 * 1024, PSKTM_E_UNSUPPORTED,
                 "traces of %d samples do not fit in LDS", p->n_samples);
+    int pad = 0;
+    if ((s = choose_row_pad(p->opt.row_pad, p->n_samples, G, (const void *)fn, sh.nth, &pad)) != PSKTM_OK)
+        return s;
+    const int nsp = pair_stride(p->n_samples, pad);
+    PSKTM_CHECK((size_t)G * nsp * sizeof(float2) <= 150 * 1024, PSKTM_E_UNSUPPORTED,
+                "row_pad %d does not fit in LDS", pad);
     if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
 
     FwdArgs a{};
@@ -347,9 +352,10 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     a.dtype = tr.dtype;
     a.G = G;
     a.nsp = nsp;
+    a.lim = p->opt.row_pad < 0 ? 0 : nsp - kPairFront;
     a.accumulate = accumulate;
     a.traces = tr.data; a.scale = tr.scale; a.slow = p->slow; a.tsq = p->tsq; a.image = image;
-    const size_t lds = (size_t)G * slot;
+    const size_t lds = (size_t)G * nsp * sizeof(float2);
     { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }
 
     // Split-K: when the image has too few (tile, tau pass) work units to fill

@@ -70,6 +70,26 @@ __global__ void assign_cells_kernel(int n, const float *__restrict__ mx,
     vals[k] = k;
 }
 
+// reach = max over traces of min(R, distance from the midpoint to the farthest
+// image corner) + |h|: no source or receiver of a trace is farther than that
+// from any bin the trace is migrated into (|s - x| <= |m - x| + |h|, and
+// |m - x| <= R inside the aperture). Evaluated in FP64 and rounded up, then
+// maxed over the bit patterns of the non-negative floats (a NaN coordinate
+// gives a NaN pattern above every finite one, which no bound test passes).
+__global__ void reach_kernel(int n, const float4 *__restrict__ geo, double cx0, double cx1,
+                             double cy0, double cy1, double R, unsigned *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 g = geo[k];
+    const double fx = fmax(fabs((double)g.x - cx0), fabs((double)g.x - cx1));
+    const double fy = fmax(fabs((double)g.y - cy0), fabs((double)g.y - cy1));
+    const double far = sqrt(fx * fx + fy * fy);
+    const double h = sqrt((double)g.z * g.z + (double)g.w * g.w);
+    const float r = (float)((fmin(R, far) + h) * (1.0 + 1e-6));
+    atomicMax(out, __float_as_uint(r));
+}
+
 __global__ void fill_int_kernel(int *p, long long n, int v)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,7 +245,7 @@ static void plan_free(psktm_plan *p)
 {
     if (!p) return;
     void *bufs[] = {p->geo, p->sorted_ids, p->cell_start, p->tile_off, p->list,
-                    p->tile_order, p->adj_items, p->slow, p->tsq, p->partial};
+                    p->tile_order, p->adj_items, p->slow, p->tsq, p->partial, p->reach};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -284,6 +304,8 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
     PLAN_HIP(dalloc(p, &p->geo, (size_t)ntr));
     PLAN_HIP(dalloc(p, &p->sorted_ids, (size_t)ntr));
     PLAN_HIP(dalloc(p, &p->cell_start, (size_t)p->n_cells + 1));
+    PLAN_HIP(dalloc(p, &p->reach, 1));
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, st, (int *)p->reach, 1ll, 0);
     const int nb = 256;
     const int gtr = (ntr + nb - 1) / nb;
 
@@ -298,6 +320,12 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
                            s.mid_x, s.mid_y, s.off_x, s.off_y, L, p->geo,
                            (int *)keys.p, (int *)vals.p, drop_key);
         if (check_launch("assign_cells_kernel") != PSKTM_OK) { plan_free(p); return PSKTM_E_LAUNCH; }
+        const double cx1 = (double)g.x0 + (double)(g.nx - 1) * g.dx;
+        const double cy1 = (double)g.y0 + (double)(g.ny - 1) * g.dy;
+        hipLaunchKernelGGL(reach_kernel, dim3(gtr), dim3(nb), 0, st, ntr, (const float4 *)p->geo,
+                           (double)g.x0, cx1, (double)g.y0, cy1,
+                           p->all_traces ? (double)INFINITY : (double)R, (unsigned *)p->reach);
+        if (check_launch("reach_kernel") != PSKTM_OK) { plan_free(p); return PSKTM_E_LAUNCH; }
         int end_bit = 1;
         while ((1ll << end_bit) <= drop_key) ++end_bit;
         size_t tmp_bytes = 0;
@@ -436,6 +464,8 @@ psktm_status psktm_plan_create(psktm_plan **plan, const psktm_grid *grid,
     PSKTM_CHECK(std::isfinite(opt->aperture), PSKTM_E_INVAL, "aperture must be finite");
     PSKTM_CHECK(opt->splits >= 0 && opt->splits <= 1024, PSKTM_E_INVAL,
                 "splits must be in [0, 1024]");
+    PSKTM_CHECK(opt->row_pad >= -1 && opt->row_pad <= (1 << 16), PSKTM_E_INVAL,
+                "row_pad must be in [-1, 65536]");
     PSKTM_CHECK((grid->nx == 1 || grid->dx > 0.0f) && (grid->ny == 1 || grid->dy > 0.0f),
                 PSKTM_E_INVAL, "bin spacings must be positive");
     int tx = opt->tile_x, ty = opt->tile_y;

@@ -61,7 +61,7 @@ class PsktmVelocity(ctypes.Structure):
 class PsktmOptions(ctypes.Structure):
     _fields_ = [("interp", _c_int), ("math", _c_int), ("aperture", _c_float),
                 ("tile_x", _c_int), ("tile_y", _c_int), ("traces_per_stage", _c_int),
-                ("precise", _c_int), ("splits", _c_int)]
+                ("precise", _c_int), ("splits", _c_int), ("row_pad", _c_int)]
 
 
 class PsktmSegyLayout(ctypes.Structure):

@@ -128,12 +128,12 @@ class Plan:
 
     def __init__(self, grid: Grid, survey: Survey, *, interp=INTERP_LINEAR, math=MATH_FAST,
                  aperture=0.0, tile_x=0, tile_y=0, traces_per_stage=0, precise=False,
-                 splits=0, stream=None):
+                 splits=0, row_pad=0, stream=None):
         L = _lib.load()
         self.grid, self.survey = grid, survey
         self.opt = PsktmOptions(int(interp), int(math), float(aperture), int(tile_x),
                                 int(tile_y), int(traces_per_stage), int(bool(precise)),
-                                int(splits))
+                                int(splits), int(row_pad))
         self._h = _p()
         g, s = grid.c(), survey.c()
         check(L.psktm_plan_create(ctypes.byref(self._h), ctypes.byref(g), ctypes.byref(s),
@@ -174,7 +174,7 @@ class Batch:
     """
 
     def __init__(self, gathers, survey: Survey, *, interp=INTERP_LINEAR, aperture=0.0,
-                 tile_x=0, tile_y=0, traces_per_stage=0, precise=False, stream=None):
+                 tile_x=0, tile_y=0, traces_per_stage=0, precise=False, row_pad=0, stream=None):
         L = _lib.load()
         self.gathers = [(g, int(f), int(n)) for g, f, n in gathers]
         self.survey = survey
@@ -182,7 +182,8 @@ class Batch:
         for q, (g, f, n) in enumerate(self.gathers):
             arr[q] = PsktmGather(g.c(), f, n)
         self.opt = PsktmOptions(int(interp), int(MATH_FAST), float(aperture), int(tile_x),
-                                int(tile_y), int(traces_per_stage), int(bool(precise)), 1)
+                                int(tile_y), int(traces_per_stage), int(bool(precise)), 1,
+                                int(row_pad))
         self._h = _p()
         s = survey.c()
         check(L.psktm_batch_create(ctypes.byref(self._h), arr, len(self.gathers),

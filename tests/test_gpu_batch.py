@@ -45,6 +45,21 @@ def test_batch_equals_per_gather_plans(cuda):
     assert total == batch.updates()
 
 
+def test_batch_row_padding_is_bit_identical(cuda):
+    """The persistent kernel re-clears its pair table for every work item
+    (gathers follow one another in a workgroup), so padded rows read zeros
+    past each gather's samples: images equal the always-clamped form."""
+    import torch
+    gd = small_gathers(cuda, n=8, seed=33)
+    q, sc = psktm.quantize(gd.traces, torch.float16)
+    outs = []
+    for pad in (-1, 0, 2000):
+        batch, _ = make_batch(gd, cuda, row_pad=pad)
+        outs.append(batch.migrate(q, gd.vel, scale=sc).clone())
+        torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+
+
 def test_batch_vs_fp64_oracle(cuda):
     gd = small_gathers(cuda, n=3, scale=0.05, seed=32)
     batch, _ = make_batch(gd, cuda)

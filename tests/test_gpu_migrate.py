@@ -216,3 +216,69 @@ def test_split_tile_lists_agree(cuda):
         assert rel_l2(v, imgs[1]) <= 2e-6, k
     with pytest.raises(psktm.PsktmError):
         gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, splits=-1)
+
+
+def _reach_bound(grid, mid, off, vel, dt, ap, vel3d):
+    """Host restatement of the kernel's per-tile traveltime bound, over the
+    whole image: 2 sqrt(tau'^2 + reach^2 s') maximised over tau and bins
+    (tile maxima are at most this), in samples; plus the smallest tile value
+    (over tau, for the bin of least slowness)."""
+    nx, x0, dx, ny, y0, dy, nt, tau0, dtau = grid
+    cx = np.array([x0, x0 + (nx - 1) * dx], np.float64)
+    cy = np.array([y0, y0 + (ny - 1) * dy], np.float64)
+    fx = np.max(np.abs(mid[:, 0:1].astype(np.float64) - cx[None]), 1)
+    fy = np.max(np.abs(mid[:, 1:2].astype(np.float64) - cy[None]), 1)
+    far = np.sqrt(fx * fx + fy * fy)
+    h = np.hypot(off[:, 0].astype(np.float64), off[:, 1].astype(np.float64))
+    reach = np.max((np.minimum(ap if ap > 0 else np.inf, far) + h)) * (1 + 1e-6)
+    tau = (tau0 + np.arange(nt) * dtau) / dt
+    v = vel.reshape(-1, nt) if vel3d else vel.reshape(1, nt)
+    s = 1.0 / (v.astype(np.float64) * dt) ** 2
+    xb = 2 * np.sqrt(tau[None] ** 2 + reach ** 2 * s)
+    return float(xb.max()), float(xb.max(1).min())
+
+
+@pytest.mark.parametrize("case", ["unclamped", "clamped"])
+def test_row_padding_is_bit_identical(cuda, case):
+    """Padded pair rows let a tile skip the index clamp when its traveltimes
+    are bounded inside the rows; the image must not change. row_pad -1 always
+    clamps. In the "unclamped" geometry every tile's bound lies inside the
+    default (ns/4) and the 4000-entry rows; in the "clamped" one, one trace
+    with a 10 km offset raises the plan's reach so that every tile's bound
+    exceeds the default rows but lies inside the 4000-entry rows. Both code paths are compared for FP32 and FP16
+    traces, LINEAR and TRUNC, 3-D and 1-D velocity, and the accumulate mode."""
+    import torch
+    if case == "unclamped":
+        grid, mid, off, tr, vel, dt, ap = small_3d(seed=21, ntr=1500, nx=9, ny=7)
+    else:
+        grid, mid, off, tr, vel, dt, ap = small_3d(seed=22, ntr=1500, nx=9, ny=7)
+        off[0] = (3000.0, 4000.0)
+    ns = tr.shape[1]
+    v1 = np.linspace(1500, 2600, grid[6]).astype(np.float32)
+    hi, lo = _reach_bound(grid, mid, off, vel, dt, ap, True)
+    lim_default = ((ns + ns // 4 + 17) & ~1) - 8
+    print(f"case {case}: bound max {hi:.1f}, min {lo:.1f}, ns {ns}, default rows {lim_default}",
+          flush=True)
+    assert hi * 1.0001 + 2 < ((ns + 4000 + 17) & ~1) - 8
+    if case == "unclamped":
+        assert hi * 1.0001 + 2 < lim_default
+    else:
+        assert lo > lim_default
+    q, sc = psktm.quantize(torch.as_tensor(tr).to(cuda), torch.float16)
+    for interp in (psktm.INTERP_LINEAR, psktm.INTERP_TRUNC):
+        outs = {}
+        for pad in (-1, 0, 4000):
+            plan, t = gpu_plan(cuda, grid, mid, off, ns, dt, ap, interp=interp, row_pad=pad)
+            a = psktm.migrate(plan, t(tr), t(vel))
+            b = psktm.migrate(plan, q, t(v1), scale=sc)
+            c = a.clone()
+            psktm.migrate(plan, t(tr), t(vel), image=c, accumulate=True)
+            outs[pad] = (a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy())
+            plan.close()
+        for pad in (0, 4000):
+            for x, y in zip(outs[pad], outs[-1]):
+                assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), (interp, pad)
+        assert all(np.count_nonzero(x) > x.size // 2 for x in outs[-1])
+        if interp == psktm.INTERP_LINEAR:
+            ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+            assert rel_l2(outs[0][0], ref) <= 1e-4
