@@ -210,37 +210,21 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
     // all plans share the tile size (same options)
     b->tb = b->plans[0]->tile_x * b->plans[0]->tile_y;
     const bool kh = opt->precise != 0;
-    b->shape = choose_shape(b->nt, b->tb, false, kh);
-    b->fn = batch_kernel(b->shape.nth, b->tb, b->shape.tl, kh);
-    if (!b->fn) {
-        batch_free(b);
-        psktm__set_error("no batch kernel for tile %d", b->tb);
-        return PSKTM_E_UNSUPPORTED;
-    }
-    const size_t slot = (size_t)pair_stride(b->ns) * sizeof(float2);
-    int G = opt->traces_per_stage > 0 ? opt->traces_per_stage : 4;
-    G = std::min(G, kGMax);
-    while (G > 1 && (size_t)G * slot > 96 * 1024) --G;
-    if ((size_t)G * slot > 150 * 1024) {
-        batch_free(b);
-        psktm__set_error("traces of %d samples do not fit in LDS", b->ns);
-        return PSKTM_E_UNSUPPORTED;
-    }
-    int pad = 0;
-    if ((st = choose_row_pad(opt->row_pad, b->ns, G, (const void *)b->fn, b->shape.nth, &pad)) !=
-        PSKTM_OK) {
+    psktm::FwdLaunch L{};
+    st = choose_launch(b->nt, b->ns, b->tb, false, kh, opt->traces_per_stage, opt->row_pad,
+                       [&](int nth, int tl) { return (const void *)batch_kernel(nth, b->tb, tl, kh); },
+                       &L);
+    if (st != PSKTM_OK) {
         batch_free(b);
         return st;
     }
-    b->nsp = pair_stride(b->ns, pad);
-    b->lim = opt->row_pad < 0 ? 0 : b->nsp - kPairFront;
-    if ((size_t)G * b->nsp * sizeof(float2) > 150 * 1024) {
-        batch_free(b);
-        psktm__set_error("row_pad %d does not fit in LDS", pad);
-        return PSKTM_E_UNSUPPORTED;
-    }
+    b->shape = L.shape;
+    b->fn = (BatchFn)L.fn;
+    b->nsp = L.nsp;
+    b->lim = opt->row_pad < 0 ? 0 : pair_limit(b->nsp);
+    const int G = L.G;
     b->G = G;
-    b->lds = (size_t)G * b->nsp * sizeof(float2);
+    b->lds = L.lds;
     if ((st = ensure_dynamic_lds((const void *)b->fn, b->lds)) != PSKTM_OK) {
         batch_free(b);
         return st;

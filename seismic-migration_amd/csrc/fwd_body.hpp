@@ -49,6 +49,9 @@ __host__ __device__ inline int pair_stride(int ns, int pad = 0)
     return (ns + pad + 2 * kPairFront + 1) & ~1;
 }
 
+// Entries from n = 0 to the end of a row of stride nsp.
+__host__ __device__ inline int pair_limit(int nsp) { return nsp - kPairFront; }
+
 __device__ __forceinline__ float decode16(unsigned bits, int dtype, float sc)
 {
     if (dtype == PSKTM_F16) return __half2float(__ushort_as_half((unsigned short)bits)) * sc;
@@ -302,10 +305,20 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
             const float sum = KH ? acc[b][j] - cmp[KH ? b : 0][KH ? j : 0] : acc[b][j];
             const float v = sum * w;
-            if (a.n_split > 1)
+            if (a.n_split > 1) {
                 a.partial[(size_t)split * a.image_elems + idx] = v;
-            else
+                if (KH) {
+                    // low-order part of (acc - cmp) * w: Fast2Sum residual of
+                    // the difference (|cmp| <= |acc|) and the exact product
+                    // error, so that the FP64 reduction of the splits keeps
+                    // what each split's compensated sum carried
+                    const float lo = (acc[b][j] - sum) - cmp[KH ? b : 0][KH ? j : 0];
+                    a.partial[(size_t)(a.n_split + split) * a.image_elems + idx] =
+                        fmaf(sum, w, -v) + lo * w;
+                }
+            } else {
                 a.image[idx] = a.accumulate ? a.image[idx] + v : v;
+            }
         }
     }
 }
@@ -319,13 +332,14 @@ constexpr int acc_regs(int tb, int tl, bool v3, bool kh)
 }
 constexpr int reg_budget(bool kh) { return kh ? 128 : 96; }
 
-
 constexpr int kTLSet[] = {2, 3, 4, 6, 8};
 
 struct FwdShape {
     int nth, tl, chunks;
 };
 
+// Register-only shape choice (no device query): the first 256- or 512-lane
+// shape that covers nt in one pass, else the widest that fits, in passes.
 inline FwdShape choose_shape(int nt, int tb, bool v3, bool kh)
 {
     auto fits = [&](int tl) { return acc_regs(tb, tl, v3, kh) <= reg_budget(kh); };
@@ -338,6 +352,106 @@ inline FwdShape choose_shape(int nt, int tb, bool v3, bool kh)
     for (int t : kTLSet)
         if (fits(t)) tl = t;
     return {512, tl, (nt + 512 * tl - 1) / (512 * tl)};
+}
+
+// Zero entries appended to every staged row (psktm_options.row_pad): -1 none,
+// > 0 that many, 0 the largest of ns/4, ns/8, ... (at least 16) whose LDS
+// footprint keeps the kernel's resident workgroups per CU.
+inline psktm_status choose_row_pad(int opt_pad, int ns, int G, const void *fn, int nth, int *pad)
+{
+    *pad = 0;
+    if (opt_pad < 0) return PSKTM_OK;
+    if (opt_pad > 0) { *pad = opt_pad; return PSKTM_OK; }
+    const size_t base_lds = (size_t)G * pair_stride(ns) * sizeof(float2);
+    int base = 0;
+    PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, nth, base_lds));
+    for (int cand = ns / 4; cand >= 16; cand /= 2) {
+        const size_t lds = (size_t)G * pair_stride(ns, cand) * sizeof(float2);
+        if (lds > 150 * 1024) continue;
+        int occ = 0;
+        PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nth, lds));
+        if (occ >= base) { *pad = cand; break; }
+    }
+    return PSKTM_OK;
+}
+
+// Launch configuration of the forward kernel for traces of ns samples.
+struct FwdLaunch {
+    FwdShape shape;
+    const void *fn;
+    int G, pad, nsp;
+    size_t lds;
+};
+
+// Chooses the shape, the traces per stage and the row padding from the
+// device's occupancy rules. The kernel hides the latency of its LDS round
+// trips and trace loads with other resident waves, so occupancy decides:
+//  * shape: among the shapes that cover nt in one pass and fit the register
+//    budget, most resident waves per CU, then fewest idle lanes (for 8-bin
+//    tiles with a 3-D velocity, 512 lanes x 3 samples at 79 VGPRs keep three
+//    workgroups per CU where 256 x 6 keeps one);
+//  * G (traces per stage): the largest of 5 .. 1 whose pair table keeps the
+//    register-limited workgroups per CU (measured on the 3-D workload: 5 for
+//    16-bin tiles at two workgroups per CU, 4 for 8-bin tiles at three);
+//  * pad: choose_row_pad at that G.
+// `lookup(nth, tl)` returns the kernel of that shape or nullptr. `opt_G` > 0
+// and `opt_pad` != 0 override the choices.
+template <typename Lookup>
+inline psktm_status choose_launch(int nt, int ns, int tb, bool v3, bool kh, int opt_G, int opt_pad,
+                                  Lookup lookup, FwdLaunch *out)
+{
+    const size_t slot = (size_t)pair_stride(ns) * sizeof(float2);
+    FwdShape best{0, 0, 0};
+    const void *best_fn = nullptr;
+    long long best_waves = -1, best_waste = 0;
+    for (int nth : {512, 256}) {
+        for (int tl : kTLSet) {
+            if ((long long)nth * tl < nt || acc_regs(tb, tl, v3, kh) > reg_budget(kh)) continue;
+            const void *fn = lookup(nth, tl);
+            if (!fn) continue;
+            int blocks = 0;
+            PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, nth, slot));
+            const long long waves = (long long)blocks * (nth / 64);
+            const long long waste = (long long)nth * tl - nt;
+            if (waves > best_waves || (waves == best_waves && waste < best_waste)) {
+                best = {nth, tl, 1};
+                best_fn = fn;
+                best_waves = waves;
+                best_waste = waste;
+            }
+        }
+    }
+    if (!best_fn) {
+        best = choose_shape(nt, tb, v3, kh);
+        best_fn = lookup(best.nth, best.tl);
+    }
+    PSKTM_CHECK(best_fn, PSKTM_E_UNSUPPORTED, "no forward kernel for tile %d", tb);
+    // occupancy queries and launches below may use more than 64 KiB of LDS
+    psktm_status s;
+    if ((s = ensure_dynamic_lds(best_fn, 150 * 1024)) != PSKTM_OK) return s;
+    int G = opt_G > 0 ? std::min(opt_G, kGMax) : 0;
+    if (G == 0) {
+        // rows as long as an explicit pad makes them
+        const size_t gslot = (size_t)pair_stride(ns, std::max(opt_pad, 0)) * sizeof(float2);
+        int reg_blocks = 0;
+        PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&reg_blocks, best_fn, best.nth, slot));
+        G = 1;
+        for (int g = 5; g >= 1; --g) {
+            int b = 0;
+            if ((size_t)g * gslot > 150 * 1024) continue;
+            PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, best_fn, best.nth, g * gslot));
+            if (b >= reg_blocks) { G = g; break; }
+        }
+    }
+    PSKTM_CHECK((size_t)G * slot <= 150 * 1024, PSKTM_E_UNSUPPORTED,
+                "traces of %d samples do not fit in LDS", ns);
+    int pad = 0;
+    if ((s = choose_row_pad(opt_pad, ns, G, best_fn, best.nth, &pad)) != PSKTM_OK) return s;
+    const int nsp = pair_stride(ns, pad);
+    PSKTM_CHECK((size_t)G * nsp * sizeof(float2) <= 150 * 1024, PSKTM_E_UNSUPPORTED,
+                "row_pad %d does not fit in LDS", pad);
+    *out = FwdLaunch{best, best_fn, G, pad, nsp, (size_t)G * nsp * sizeof(float2)};
+    return PSKTM_OK;
 }
 
 // Fills the FwdArgs fields that depend only on the plan.
@@ -356,29 +470,6 @@ inline void fwd_args_from_plan(const psktm_plan *p, FwdArgs &a)
     a.interp = p->opt.interp;
     a.geo = p->geo; a.list = p->list; a.tile_off = p->tile_off; a.tile_order = p->tile_order;
     a.reach = p->reach;
-}
-
-// Zero entries appended to every staged row (psktm_options.row_pad): -1 none,
-// > 0 that many, 0 the largest of ns/4, ns/8, ... (at least 16) whose LDS
-// footprint keeps the kernel's resident workgroups per CU and stays within
-// 64 KiB (or the unpadded footprint, if that is larger).
-inline psktm_status choose_row_pad(int opt_pad, int ns, int G, const void *fn, int nth, int *pad)
-{
-    *pad = 0;
-    if (opt_pad < 0) return PSKTM_OK;
-    if (opt_pad > 0) { *pad = opt_pad; return PSKTM_OK; }
-    const size_t base_lds = (size_t)G * pair_stride(ns) * sizeof(float2);
-    const size_t cap = std::max<size_t>(base_lds, 64 * 1024);
-    int base = 0;
-    PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, nth, base_lds));
-    for (int cand = ns / 4; cand >= 16; cand /= 2) {
-        const size_t lds = (size_t)G * pair_stride(ns, cand) * sizeof(float2);
-        if (lds > cap) continue;
-        int occ = 0;
-        PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nth, lds));
-        if (occ >= base) { *pad = cand; break; }
-    }
-    return PSKTM_OK;
 }
 
 } // namespace psktm

@@ -229,12 +229,21 @@ __global__ void zero_kernel(float *p, size_t n)
     if (i < n) p[i] = 0.f;
 }
 
-// image[i] (+)= sum over splits of partial[s][i], in split order.
+// image[i] (+)= sum over splits of partial[s][i], in split order. Precise
+// mode (compensated != 0): partial also holds n_split low-order planes after
+// the n_split high-order ones, and everything is added in FP64 and rounded once.
 __global__ void reduce_partials_kernel(const float *__restrict__ partial, int n_split,
-                                       long long n, int accumulate, float *__restrict__ image)
+                                       long long n, int accumulate, int compensated,
+                                       float *__restrict__ image)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (compensated) {
+        double t = accumulate ? (double)image[i] : 0.0;
+        for (int s = 0; s < 2 * n_split; ++s) t += (double)partial[(size_t)s * n + i];
+        image[i] = (float)t;
+        return;
+    }
     float t = partial[i];
     for (int s = 1; s < n_split; ++s) t += partial[(size_t)s * n + i];
     image[i] = accumulate ? image[i] + t : t;
@@ -328,22 +337,15 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     const int tb = p->tile_x * p->tile_y;
     const bool v3 = vel.kind == PSKTM_VEL_3D;
     const bool kh = p->opt.precise != 0;
-    const FwdShape sh = choose_shape(g.nt, tb, v3, kh);
-    FwdFn fn = fwd_kernel(sh.nth, tb, sh.tl, v3, kh);
-    PSKTM_CHECK(fn, PSKTM_E_UNSUPPORTED, "no forward kernel for tile %d tl %d", tb, sh.tl);
-
-    const size_t slot = (size_t)pair_stride(p->n_samples) * sizeof(float2);
-    int G = p->opt.traces_per_stage > 0 ? p->opt.traces_per_stage : 4;
-    G = std::min(G, kGMax);
-    while (G > 1 && (size_t)G * slot > 96 * 1024) --G;
-    PSKTM_CHECK((size_t)G * slot <= 150 * 1024, PSKTM_E_UNSUPPORTED,
-                "traces of %d samples do not fit in LDS", p->n_samples);
-    int pad = 0;
-    if ((s = choose_row_pad(p->opt.row_pad, p->n_samples, G, (const void *)fn, sh.nth, &pad)) != PSKTM_OK)
+    FwdLaunch L{};
+    if ((s = choose_launch(g.nt, p->n_samples, tb, v3, kh, p->opt.traces_per_stage,
+                           p->opt.row_pad,
+                           [&](int nth, int tl) { return (const void *)fwd_kernel(nth, tb, tl, v3, kh); },
+                           &L)) != PSKTM_OK)
         return s;
-    const int nsp = pair_stride(p->n_samples, pad);
-    PSKTM_CHECK((size_t)G * nsp * sizeof(float2) <= 150 * 1024, PSKTM_E_UNSUPPORTED,
-                "row_pad %d does not fit in LDS", pad);
+    const FwdShape sh = L.shape;
+    const FwdFn fn = (FwdFn)L.fn;
+    const int G = L.G, nsp = L.nsp;
     if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
 
     FwdArgs a{};
@@ -352,10 +354,10 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     a.dtype = tr.dtype;
     a.G = G;
     a.nsp = nsp;
-    a.lim = p->opt.row_pad < 0 ? 0 : nsp - kPairFront;
+    a.lim = p->opt.row_pad < 0 ? 0 : pair_limit(nsp);
     a.accumulate = accumulate;
     a.traces = tr.data; a.scale = tr.scale; a.slow = p->slow; a.tsq = p->tsq; a.image = image;
-    const size_t lds = (size_t)G * nsp * sizeof(float2);
+    const size_t lds = L.lds;
     { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }
 
     // Split-K: when the image has too few (tile, tau pass) work units to fill
@@ -381,7 +383,7 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     a.n_split = n_split;
     a.image_elems = (long long)g.nx * g.ny * g.nt;
     if (n_split > 1) {
-        const size_t need = (size_t)n_split * (size_t)a.image_elems;
+        const size_t need = (size_t)n_split * (size_t)a.image_elems * (kh ? 2 : 1);
         if (p->partial_cap < need) {
             if (p->partial) {
                 PSKTM_HIP(hipStreamSynchronize(st));
@@ -405,7 +407,7 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     if (n_split > 1) {
         const long long n = a.image_elems;
         hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           st, (const float *)p->partial, n_split, n, accumulate, image);
+                           st, (const float *)p->partial, n_split, n, accumulate, kh ? 1 : 0, image);
         return check_launch("reduce_partials_kernel");
     }
     return PSKTM_OK;

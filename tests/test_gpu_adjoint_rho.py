@@ -17,15 +17,20 @@ def dot64(a, b):
     return float(np.sum(np.asarray(a, np.float64) * np.asarray(b, np.float64)))
 
 
-@pytest.mark.parametrize("precise,tol", [(True, 1e-6), (False, 2e-6)])
+@pytest.mark.parametrize("precise,tol", [(True, 1e-6), (False, 5e-6)])
 @pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
 def test_dot_product_test_3d(cuda, interp, precise, tol):
     """<F d, m> = <d, F' m> with zero-mean random d, m and FP64 inner products.
 
     The adjoint accumulates in FP64 in both modes. Precise mode (Kahan
-    forward) meets 1e-6. The plain forward's FP32 sum of K ~ 2000 random-sign
-    terms per image sample has a relative error ~ eps*sqrt(K/2) ~ 2e-6 per
-    sample; summed over many samples the test sees ~4e-7, bounded here by 2e-6.
+    forward, split-K partials reduced with their low-order parts in FP64)
+    meets 1e-6. The plain forward's FP32 sum of K ~ 2000 random-sign terms per
+    image sample has a relative error ~ eps*sqrt(K/2) ~ 2e-6 per sample, and
+    |<d,F'm>| here is a small fraction of the sum of its absolute terms, so the
+    plain-mode figure depends on the summation order, i.e. on the tile shape:
+    6.5e-7 (4x4 bins), 2.9e-6 (4x2), 3.7e-6 (2x4), 2.4e-6 (8x2) on this
+    survey, bounded here by 5e-6. test_dot_product_test_normalised bounds both
+    modes against the sum of absolute terms instead.
     """
     import torch
     grid, mid, off, tr, vel, dt, ap = small_3d(seed=11, ntr=2000, nx=10, ny=9)
