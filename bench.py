@@ -10,15 +10,18 @@ cos(y/3000); 200 point diffractors (30 Hz Ricker) plus Gaussian noise at
 
 One step = one pass of the hot path over the resident survey: build the
 plan (midpoint binning and per-tile aperture lists), rho-filter the traces
-(2.5-D half derivative), quantise them to FP16 with per-trace FP32 scales,
-and migrate (LINEAR interpolation, FAST math, FP32 accumulation). The plain
-FP32 form is the same pass with FP32 traces and no quantisation; its rate is
-reported beside the headline for the FP16/FP32 ratio.
+(2.5-D half derivative) with the trace-domain part of the weight (sample n
+times sqrt(n dt)) fused in, quantise them to FP16 with per-trace FP32
+scales, and migrate (LINEAR interpolation, FAST math, FP32 accumulation,
+weight tau per update). The FP16 form with the per-update weight tau*sqrt(t)
+(no gain) and the plain FP32 form (FP32 traces, per-update weight) are timed
+the same way and reported beside the headline.
 
-value = updates / s of the FP16 form, where one update is one (trace,
-in-aperture bin, output sample) triple counted by the plan. The image of the
-FP16, BF16 and FP32 forms is compared with the FP64 oracle on a central
-sub-volume of bins (rel-L2). The reference's CPU loop (the oracle's binary32
+value = updates / s of the headline form, where one update is one (trace,
+in-aperture bin, output sample) triple counted by the plan. Every form's
+image is compared with the FP64 oracle of the per-update weight (the
+reference's operator with LINEAR interpolation) on a central sub-volume of
+bins (rel-L2). The reference's CPU loop (the oracle's binary32
 restatement of src/psktmCPU.c) is timed on a bounded sample of the same
 workload on the host cores. Only rank 0 of one GPU runs.
 """
@@ -43,8 +46,14 @@ METRIC = "G trace-sample updates/s: FP16-trace vs FP32 plain, rel-L2≤1e-3 vs F
 # compiled without the clamp (11 issued instructions per update); the
 # roofline keeps the algorithm's 12.
 VALU_OPS_PER_UPDATE = 12
+# The gained-trace form (sqrt(t) applied per trace sample by the rho filter)
+# has no per-update weight sqrt: 11 algorithmic VALU instructions, 2 of them
+# v_sqrt_f32.
+VALU_OPS_PER_UPDATE_GAINED = 11
+FORM_DTYPES = {"f16g": "f16", "f16": "f16", "bf16": "bf16", "f32": "f32", "f32g": "f32"}
 # Conventional FLOP count per update (FMA = 2, sqrt = 1, add = 1).
 FLOPS_PER_UPDATE = 12
+FLOPS_PER_UPDATE_GAINED = 10      # 2 FMA, 2 sqrt, add, interpolation FMA, add
 # MI355X: 256 CUs, 2.4 GHz, 64 FP32 lanes per CU per clock without packed
 # math; published FP32 vector peak (packed FMA) 157.3 TFLOP/s.
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
@@ -68,7 +77,9 @@ def parse():
                     help="traces in the CPU-baseline sample")
     ap.add_argument("--oracle-bins", type=int, default=4,
                     help="side of the central sub-volume compared with FP64")
-    ap.add_argument("--forms", default="f16,f32", help="timed forms (f16, f32, bf16)")
+    ap.add_argument("--forms", default="f16g,f16,f32",
+                    help="timed forms: f16g (FP16, gained traces; the headline), f16, "
+                         "bf16, f32 (FP32 plain), f32g; the first is the headline")
     ap.add_argument("--workload", default="pstm3d", choices=["pstm3d", "batched", "lsm", "line2d"],
                     help="pstm3d: the headline 3-D workload (default); batched: 64 offset-class "
                          "gathers in one persistent launch; lsm: least-squares 2-D line, 20 CG "
@@ -148,15 +159,16 @@ def main():
 
     def one_step(form, ev=None):
         plan = psktm.Plan(grid, survey, aperture=s.aperture)
-        if form == "f32":
-            tr = psktm.rho_filter(s.traces, dw_us, 2)
+        gained = form.endswith("g")
+        if FORM_DTYPES[form] == "f32":
+            tr = psktm.rho_filter(s.traces, dw_us, 2, gain=gained)
             sc = None
         else:
-            dtype = torch.float16 if form == "f16" else torch.bfloat16
-            tr, sc = psktm.rho_filter(s.traces, dw_us, 2, out_dtype=dtype)
+            dtype = torch.float16 if FORM_DTYPES[form] == "f16" else torch.bfloat16
+            tr, sc = psktm.rho_filter(s.traces, dw_us, 2, out_dtype=dtype, gain=gained)
         if ev is not None:
             ev[0].record(stream)
-        psktm.migrate(plan, tr, s.vel, image, scale=sc)
+        psktm.migrate(plan, tr, s.vel, image, scale=sc, gained=gained)
         if ev is not None:
             ev[1].record(stream)
         return plan, tr, sc
@@ -221,13 +233,16 @@ def main():
             log(f"FP64 oracle: {r1}/{nb} bin rows ({time.time() - t1:.0f} s)")
         log(f"FP64 oracle on {nb}x{nb} bins x {grid.nt} ({len(sel)} traces) in {time.time() - t1:.1f} s")
         plan = psktm.Plan(grid, survey, aperture=s.aperture)
-        for form in ("f32", "f16", "bf16"):
-            if form == "f32":
-                img = psktm.migrate(plan, f32_traces, s.vel)
+        for form in ("f16g", "f32g", "f32", "f16", "bf16"):
+            gained = form.endswith("g")
+            if FORM_DTYPES[form] == "f32":
+                src = psktm.trace_gain(f32_traces, dt) if gained else f32_traces
+                img = psktm.migrate(plan, src, s.vel, gained=gained)
             else:
-                dtype = torch.float16 if form == "f16" else torch.bfloat16
-                q, sc = psktm.quantize(f32_traces, dtype)
-                img = psktm.migrate(plan, q, s.vel, scale=sc)
+                dtype = torch.float16 if FORM_DTYPES[form] == "f16" else torch.bfloat16
+                q, sc = psktm.trace_gain(f32_traces, dt, dtype) if gained else \
+                    psktm.quantize(f32_traces, dtype)
+                img = psktm.migrate(plan, q, s.vel, scale=sc, gained=gained)
             sub = img[bins[0]:bins[1], bins[2]:bins[3]].double().cpu().numpy()
             acc[form] = dict(rel_l2=float(np.linalg.norm(sub - ref) / np.linalg.norm(ref)),
                              max_abs=float(np.max(np.abs(sub - ref))),
@@ -267,18 +282,24 @@ def main():
 
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # runs of this same workload (scaled by the copy-kernel calibration)
+    head_form = next(iter(results))
+    head = results[head_form]
     traffic = None
     tp = os.path.join(ROOT, "profiles", "migrate_fwd_3d_1M_traces_traffic.json")
-    if os.path.exists(tp) and "f16" in results:
+    if os.path.exists(tp):
         prof = json.load(open(tp))
         if prof["workload"]["traces"] == args.traces and \
+                prof["workload"].get("form") == head_form and \
                 abs(prof["updates_per_launch"] - updates) <= 1e-9 * updates:
             traffic = prof["hbm_bytes_per_launch"]
 
-    head = results.get("f16") or next(iter(results.values()))
     f32 = results.get("f32")
+    f16 = results.get("f16")
     kern_s = head["kernel_ms"] / 1e3
-    valu_ops = updates * VALU_OPS_PER_UPDATE / kern_s
+    per_update = VALU_OPS_PER_UPDATE_GAINED if head_form.endswith("g") else VALU_OPS_PER_UPDATE
+    valu_ops = updates * per_update / kern_s
+    flops = updates * (FLOPS_PER_UPDATE_GAINED if head_form.endswith("g")
+                       else FLOPS_PER_UPDATE) / kern_s
     line = {
         "metric": METRIC,
         "value": head["rate"] / 1e9,
@@ -294,15 +315,20 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "3-D PSTM: 1e6 traces x 1500 samples (4 ms) -> 200x200 bins x 1500 "
-                        "tau, aperture 1500 m, v(x,y,t0); step = plan + rho filter + FP16 "
-                        "quantise + migration (LINEAR, FAST)",
+                        "tau, aperture 1500 m, v(x,y,t0); step = plan + rho filter (fused "
+                        "sqrt(n dt) gain for gained forms) + quantise + migration (LINEAR, "
+                        "FAST)",
             "traces": args.traces, "samples": ns, "image": list(grid.shape),
-            "aperture_m": s.aperture, "trace_storage": "f16 + per-trace f32 scale",
+            "aperture_m": s.aperture, "headline_form": head_form,
+            "trace_storage": {"f16g": "f16 + per-trace f32 scale, samples gained by sqrt(n dt)",
+                              "f16": "f16 + per-trace f32 scale", "bf16": "bf16 + per-trace "
+                              "f32 scale", "f32": "f32", "f32g": "f32, gained"}[head_form],
             "updates_per_step": updates,
         },
         "forms": {k: {"G_updates_per_s": v["rate"] / 1e9, "ms_per_step": v["ms_per_step"],
                       "migrate_kernel_ms": v["kernel_ms"]} for k, v in results.items()},
         "fp16_over_fp32": (head["rate"] / f32["rate"]) if f32 else None,
+        "fp16_over_fp32_same_weight": (f16["rate"] / f32["rate"]) if (f16 and f32) else None,
         "accuracy_vs_fp64": acc or None,
         "roofline": {
             "bound": "valu",
@@ -314,11 +340,12 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
             "hbm_GBps": (traffic / kern_s / 1e9) if traffic else None,
             "kernel": "migrate_fwd_kernel",
-            "per_update": f"{VALU_OPS_PER_UPDATE} VALU instructions (3 v_sqrt_f32; 11 issued "
-                          f"where the index clamp is not needed)",
-            "fp32_published": {"achieved_tflops": updates * FLOPS_PER_UPDATE / kern_s / 1e12,
+            "form": head_form,
+            "per_update": (f"{per_update} VALU instructions ({3 - (per_update < 12)} v_sqrt_f32; "
+                           f"{per_update - 1} issued where the index clamp is not needed)"),
+            "fp32_published": {"achieved_tflops": flops / 1e12,
                                "peak_tflops": FP32_PUBLISHED_TFLOPS,
-                               "frac": updates * FLOPS_PER_UPDATE / kern_s / 1e12 / FP32_PUBLISHED_TFLOPS},
+                               "frac": flops / 1e12 / FP32_PUBLISHED_TFLOPS},
         },
         "cpu_baseline": cpu,
     }

@@ -169,6 +169,17 @@ typedef struct psktm_traces {
     int          dtype;       /* psktm_dtype */
     const void  *data;
     const float *scale;       /* [n_traces]; NULL for F32 */
+    int          gained;      /* 0: raw samples; the forward applies the weight
+                                 t*sqrt(tau^2/t) = tau*sqrt(t) per update
+                                 (src/psktmCPU.c:84-85). 1: sample n already
+                                 carries sqrt(n*dt) (psktm_rho_filter_gain,
+                                 psktm_trace_gain), and the forward applies
+                                 tau only: one v_sqrt_f32 fewer per update.
+                                 The interpolated gained sample differs from
+                                 the interpolated sample times sqrt(t) by
+                                 f(1-f)(g[n+1]-g[n]) terms, ~1e-4 relative on
+                                 30 Hz data at 4 ms. psktm_demigrate applies
+                                 the per-update weight in either case. */
 } psktm_traces;
 
 /* RMS velocity in m/s on the device: VEL_1D -> v[nt] (laterally invariant, as
@@ -287,6 +298,20 @@ psktm_status psktm_rho_filter(const float *in, int n_traces, int n_samples,
                               float dw_us, int opt, int out_dtype, void *out,
                               float *scale_out, int matlab_trace_shift,
                               void *stream);
+/* psktm_rho_filter followed, in the same kernel, by the trace-domain part of
+ * the migration weight: output sample n is the filtered value times
+ * sqrtf(n * gain_dt) (gain_dt = the sample interval in seconds; 0 = no gain,
+ * the same output as psktm_rho_filter), quantised after the gain. Traces
+ * written with a gain are migrated with psktm_traces.gained = 1. */
+psktm_status psktm_rho_filter_gain(const float *in, int n_traces, int n_samples,
+                                   float dw_us, int opt, int out_dtype, void *out,
+                                   float *scale_out, int matlab_trace_shift,
+                                   float gain_dt, void *stream);
+/* The gain alone: out[k][n] = in[k][n] * sqrtf(n * gain_dt) (each product
+ * rounded once), as F32, or F16/BF16 with per-trace max-abs scales. */
+psktm_status psktm_trace_gain(const float *in, int n_traces, int n_samples,
+                              float gain_dt, int out_dtype, void *out,
+                              float *scale_out, void *stream);
 /* FP32 -> F16/BF16 with per-trace max-abs scales, no filtering. */
 psktm_status psktm_quantize(const float *in, int n_traces, int n_samples,
                             int out_dtype, void *out, float *scale_out,

@@ -45,7 +45,7 @@ def lib():
                                                 _p, _p, _p, _p, _p, _p, _p, _i]
         L.oracle_ktm_migration_racy.restype = None
         L.oracle_pstm_f64.argtypes = [_i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _i, _f,
-                                      _p, _p, _p, _p, _p, _p, _p, _i, _f, _i,
+                                      _p, _p, _p, _p, _p, _p, _p, _i, _f, _i, _i,
                                       _i, _i, _i, _i, _p, _i]
         L.oracle_pstm_f64.restype = None
         L.oracle_pstm_adj_f64.argtypes = [_i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _i, _f,
@@ -124,8 +124,11 @@ TRUNC, LINEAR = 0, 1
 
 
 def pstm_f64(grid: Grid, mid_x, mid_y, off_x, off_y, traces, vel, dt, *, vel3d=False,
-             aperture=0.0, interp=LINEAR, scale=None, bins=None, threads=0):
-    """FP64 forward operator on bins ``(ix0, ix1, iy0, iy1)`` (default: all)."""
+             aperture=0.0, interp=LINEAR, scale=None, bins=None, gained=False, threads=0):
+    """FP64 forward operator on bins ``(ix0, ix1, iy0, iy1)`` (default: all).
+
+    ``gained``: the traces carry sqrt(n dt) per sample and the amplitude
+    weight is tau alone (the GPU's gained-trace form)."""
     traces = np.ascontiguousarray(traces, dtype=np.float32)
     ntr, ns = traces.shape
     ix0, ix1, iy0, iy1 = bins if bins is not None else (0, grid.nx, 0, grid.ny)
@@ -142,7 +145,7 @@ def pstm_f64(grid: Grid, mid_x, mid_y, off_x, off_y, traces, vel, dt, *, vel3d=F
         sp = None
     va, vp = _c(vel, np.float32)
     lib().oracle_pstm_f64(*grid.args(), ntr, ns, float(dt), *ptrs, sp, vp,
-                          int(bool(vel3d)), float(aperture), int(interp),
+                          int(bool(vel3d)), float(aperture), int(interp), int(bool(gained)),
                           int(ix0), int(ix1), int(iy0), int(iy1),
                           out.ctypes.data_as(_p), int(threads))
     return out

@@ -17,7 +17,8 @@
  *                              build adds (aperture, v(x,y,tau), trace range).
  *   oracle_ktm_migration_racy  the reference's trace-parallel loop structure
  *                              as written (racy; timing only).
- *   oracle_pstm_f64            FP64 forward (TRUNC or LINEAR interpolation).
+ *   oracle_pstm_f64            FP64 forward (TRUNC or LINEAR interpolation;
+ *                              per-update weight or gained traces).
  *   oracle_pstm_adj_f64        FP64 adjoint (demigration) of the same operator.
  *   oracle_init_three_axes     restatement of initThreeAxes
  *                              (src/psktmFunctions.c:19-39).
@@ -237,9 +238,10 @@ void oracle_ktm_migration_racy(int nx, float x0, float dx, int ny, float y0,
  *   TRUNC : n = floor(t/dt), valid iff 0 < n < ns, weights (1, 0)
  *   LINEAR: n = floor(t/dt), f = t/dt - n, valid iff 0 <= n and n+1 < ns,
  *           weights (1-f, f) on samples n, n+1
- *   amplitude weight t*sqrt(tau^2/t) = tau*sqrt(t) (src/psktmCPU.c:84-85). */
+ *   amplitude weight t*sqrt(tau^2/t) = tau*sqrt(t) (src/psktmCPU.c:84-85);
+ *   gained != 0: tau only (the traces carry sqrt(n dt) per sample). */
 static inline int update_f64(double tau, double s, double A, double B,
-                             double dt, int ns, int interp, long *n_out,
+                             double dt, int ns, int interp, int gained, long *n_out,
                              double *w0, double *w1)
 {
     double tau2 = tau * tau;
@@ -247,7 +249,7 @@ static inline int update_f64(double tau, double s, double A, double B,
     double xi = t / dt;
     if (!(xi >= 0.0) || xi >= (double)ns) return 0;
     long n = (long)floor(xi);
-    double amp = (t > 0.0) ? t * sqrt(tau2 / t) : 0.0;
+    double amp = gained ? tau : ((t > 0.0) ? t * sqrt(tau2 / t) : 0.0);
     if (interp == PSKTM_INTERP_TRUNC) {
         if (!(n > 0 && n < ns)) return 0;
         *n_out = n; *w0 = amp; *w1 = 0.0;
@@ -284,7 +286,7 @@ void oracle_pstm_f64(int nx, float x0, float dx, int ny, float y0, float dy,
                      const float *midX, const float *midY, const float *offX,
                      const float *offY, const float *traces, const float *scale,
                      const float *vel, int vel3d, float aperture, int interp,
-                     int ix0, int ix1, int iy0, int iy1, double *out,
+                     int gained, int ix0, int ix1, int iy0, int iy1, double *out,
                      int nthreads)
 {
     float *xx = (float *)malloc(sizeof(float) * (size_t)nx);
@@ -317,7 +319,7 @@ void oracle_pstm_f64(int nx, float x0, float dx, int ny, float y0, float dy,
             const double sc = scale ? (double)scale[k] : 1.0;
             for (int l = 0; l < nt; l++) {
                 long n; double w0, w1;
-                if (!update_f64(zz[l], sb[l], A, B, dt, ns, interp, &n, &w0, &w1))
+                if (!update_f64(zz[l], sb[l], A, B, dt, ns, interp, gained, &n, &w0, &w1))
                     continue;
                 double d = w0 * (double)tr[n];
                 if (w1 != 0.0) d += w1 * (double)tr[n + 1];
@@ -367,7 +369,7 @@ void oracle_pstm_adj_f64(int nx, float x0, float dx, int ny, float y0, float dy,
                 const double *sb = vel3d ? s + ((size_t)i * ny + j) * nt : s;
                 for (int l = 0; l < nt; l++) {
                     long n; double w0, w1;
-                    if (!update_f64(zz[l], sb[l], A, B, dt, ns, interp, &n, &w0, &w1))
+                    if (!update_f64(zz[l], sb[l], A, B, dt, ns, interp, 0, &n, &w0, &w1))
                         continue;
                     tr[n] += w0 * (double)im[l];
                     if (w1 != 0.0) tr[n + 1] += w1 * (double)im[l];

@@ -21,7 +21,7 @@
 namespace psktm {
 namespace {
 
-template <int NTH, int TB, int TL, bool KH>
+template <int NTH, int TB, int TL, bool KH, bool GN>
 __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__restrict__ descs,
                                                             const int2 *__restrict__ items,
                                                             int n_items,
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__res
         const int g = __builtin_amdgcn_readfirstlane(items[it].x);
         const int tc = __builtin_amdgcn_readfirstlane(items[it].y);
         const FwdArgs a = descs[g];
-        fwd_tile<NTH, TB, TL, false, KH>(a, tc / a.n_chunks, tc % a.n_chunks, 0, s_pairs,
+        fwd_tile<NTH, TB, TL, false, KH, GN>(a, tc / a.n_chunks, tc % a.n_chunks, 0, s_pairs,
                                          s_ab, s_mask);
     }
 }
@@ -61,36 +61,39 @@ __global__ void batch_tables_kernel(int nt, float tau0, float dtau, float inv_dt
 
 using BatchFn = void (*)(const FwdArgs *, const int2 *, int, unsigned *);
 
-template <int NTH, int TB, int TL, bool KH>
+template <int NTH, int TB, int TL, bool KH, bool GN>
 BatchFn batch_pick()
 {
-    if constexpr (acc_regs(TB, TL, false, KH) <= reg_budget(KH))
-        return migrate_batch_kernel<NTH, TB, TL, KH>;
+    if constexpr (acc_regs(TB, TL, false, KH) <= reg_budget(KH) && !(KH && GN))
+        return migrate_batch_kernel<NTH, TB, TL, KH, GN>;
     else
         return nullptr;
 }
 
-template <int NTH, int TB, bool KH>
+template <int NTH, int TB, bool KH, bool GN>
 BatchFn batch_by_tl(int tl)
 {
     switch (tl) {
-    case 2: return batch_pick<NTH, TB, 2, KH>();
-    case 3: return batch_pick<NTH, TB, 3, KH>();
-    case 4: return batch_pick<NTH, TB, 4, KH>();
-    case 6: return batch_pick<NTH, TB, 6, KH>();
-    case 8: return batch_pick<NTH, TB, 8, KH>();
+    case 2: return batch_pick<NTH, TB, 2, KH, GN>();
+    case 3: return batch_pick<NTH, TB, 3, KH, GN>();
+    case 4: return batch_pick<NTH, TB, 4, KH, GN>();
+    case 6: return batch_pick<NTH, TB, 6, KH, GN>();
+    case 8: return batch_pick<NTH, TB, 8, KH, GN>();
     default: return nullptr;
     }
 }
 
-BatchFn batch_kernel(int nth, int tb, int tl, bool kh)
+template <int NTH, int TB>
+BatchFn batch_by_mode(int tl, bool kh, bool gn)
 {
-    if (nth == 256) {
-        if (tb == 8) return kh ? batch_by_tl<256, 8, true>(tl) : batch_by_tl<256, 8, false>(tl);
-        return kh ? batch_by_tl<256, 16, true>(tl) : batch_by_tl<256, 16, false>(tl);
-    }
-    if (tb == 8) return kh ? batch_by_tl<512, 8, true>(tl) : batch_by_tl<512, 8, false>(tl);
-    return kh ? batch_by_tl<512, 16, true>(tl) : batch_by_tl<512, 16, false>(tl);
+    if (kh) return batch_by_tl<NTH, TB, true, false>(tl);
+    return gn ? batch_by_tl<NTH, TB, false, true>(tl) : batch_by_tl<NTH, TB, false, false>(tl);
+}
+
+BatchFn batch_kernel(int nth, int tb, int tl, bool kh, bool gn)
+{
+    if (nth == 256) return tb == 8 ? batch_by_mode<256, 8>(tl, kh, gn) : batch_by_mode<256, 16>(tl, kh, gn);
+    return tb == 8 ? batch_by_mode<512, 8>(tl, kh, gn) : batch_by_mode<512, 16>(tl, kh, gn);
 }
 
 } // namespace
@@ -104,7 +107,8 @@ struct psktm_batch {
     int ns = 0, nt = 0, tb = 0;
     float dt = 0.f, tau0 = 0.f, dtau = 0.f;
     psktm::FwdShape shape{};
-    psktm::BatchFn fn = nullptr;
+    psktm::BatchFn fn = nullptr;        // raw traces
+    psktm::BatchFn fn_gained = nullptr; // gained traces (same shape), NULL in precise mode
     int G = 4, nsp = 0, lim = 0;
     size_t lds = 0;
     int blocks = 0;
@@ -138,7 +142,7 @@ void batch_free(psktm_batch *b)
     delete b;
 }
 
-psktm_status enqueue(psktm_batch *b, const float *v, hipStream_t st)
+psktm_status enqueue(psktm_batch *b, const float *v, bool gained, hipStream_t st)
 {
     const double inv_dt = 1.0 / (double)b->dt;
     hipLaunchKernelGGL(batch_tables_kernel, dim3((b->nt + 255) / 256), dim3(256), 0, st, b->nt,
@@ -146,7 +150,8 @@ psktm_status enqueue(psktm_batch *b, const float *v, hipStream_t st)
                        b->tsq);
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, b->d_counter);
     if (b->n_items > 0)
-        hipLaunchKernelGGL(b->fn, dim3(b->blocks), dim3(b->shape.nth), b->lds, st,
+        hipLaunchKernelGGL(gained ? b->fn_gained : b->fn, dim3(b->blocks), dim3(b->shape.nth),
+                           b->lds, st,
                            (const FwdArgs *)b->d_args, (const int2 *)b->d_items, b->n_items,
                            b->d_counter);
     return check_launch("migrate_batch_kernel");
@@ -212,7 +217,7 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
     const bool kh = opt->precise != 0;
     psktm::FwdLaunch L{};
     st = choose_launch(b->nt, b->ns, b->tb, false, kh, opt->traces_per_stage, opt->row_pad,
-                       [&](int nth, int tl) { return (const void *)batch_kernel(nth, b->tb, tl, kh); },
+                       [&](int nth, int tl) { return (const void *)batch_kernel(nth, b->tb, tl, kh, false); },
                        &L);
     if (st != PSKTM_OK) {
         batch_free(b);
@@ -220,12 +225,14 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
     }
     b->shape = L.shape;
     b->fn = (BatchFn)L.fn;
+    b->fn_gained = kh ? nullptr : batch_kernel(L.shape.nth, b->tb, L.shape.tl, false, true);
     b->nsp = L.nsp;
     b->lim = opt->row_pad < 0 ? 0 : pair_limit(b->nsp);
     const int G = L.G;
     b->G = G;
     b->lds = L.lds;
-    if ((st = ensure_dynamic_lds((const void *)b->fn, b->lds)) != PSKTM_OK) {
+    if ((st = ensure_dynamic_lds((const void *)b->fn, b->lds)) != PSKTM_OK ||
+        (b->fn_gained && (st = ensure_dynamic_lds((const void *)b->fn_gained, b->lds)) != PSKTM_OK)) {
         batch_free(b);
         return st;
     }
@@ -307,9 +314,12 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
                 PSKTM_E_INVAL, "unknown trace dtype %d", tr->dtype);
     PSKTM_CHECK(tr->data && (tr->dtype == PSKTM_F32 || tr->scale), PSKTM_E_INVAL,
                 "trace data or scales missing");
+    PSKTM_CHECK(tr->gained == 0 || tr->gained == 1, PSKTM_E_INVAL, "gained must be 0 or 1");
+    PSKTM_CHECK(!tr->gained || b->fn_gained, PSKTM_E_UNSUPPORTED,
+                "precise mode takes raw traces (gained = 0)");
     hipStream_t st = as_stream(stream);
     const void *key[5] = {tr->data, tr->scale, vel->v, images,
-                          (const void *)(intptr_t)(tr->dtype + 1)};
+                          (const void *)(intptr_t)(tr->dtype + 1 + 16 * tr->gained)};
     const bool same = b->exec && std::equal(key, key + 5, b->key);
     if (use_graph && same) {
         PSKTM_HIP(hipGraphLaunch(b->exec, st));
@@ -323,6 +333,8 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
         fwd_args_from_plan(b->plans[g], a);
         a.n_chunks = b->shape.chunks;
         a.dtype = tr->dtype;
+        a.gained = tr->gained;
+        if (tr->gained) a.wscale = 1.0f;
         a.G = b->G;
         a.nsp = b->nsp;
         a.lim = b->lim;
@@ -340,7 +352,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
                                  hipMemcpyHostToDevice, st));
         // h_args must outlive the copy: wait for it before returning
         PSKTM_HIP(hipStreamSynchronize(st));
-        return enqueue(b, vel->v, st);
+        return enqueue(b, vel->v, tr->gained != 0, st);
     }
     // (re)capture for these pointers. Recording happens on a private
     // non-blocking stream (the caller's stream may be the legacy default
@@ -353,7 +365,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
     PSKTM_HIP(hipMemcpy(b->d_args, b->h_args.data(), sizeof(FwdArgs) * b->h_args.size(),
                         hipMemcpyHostToDevice));
     PSKTM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    psktm_status s = enqueue(b, vel->v, b->cap);
+    psktm_status s = enqueue(b, vel->v, tr->gained != 0, b->cap);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap, &graph);
     if (s != PSKTM_OK) {

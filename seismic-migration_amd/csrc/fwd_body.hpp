@@ -23,6 +23,7 @@ struct FwdArgs {
     int n_split;                  // workgroups per (tile, tau pass); > 1 writes partials
     float *partial;               // [n_split][image] when n_split > 1
     long long image_elems;
+    int gained;                   // traces carry sqrt(n dt): no per-update weight
     int lim;                      // pair-table entries from n = 0 in every staged row
     const float *reach;           // plan's bound on |s - x| and |r - x| (device scalar)
     const float4 *geo;
@@ -153,9 +154,11 @@ __device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, fl
 }
 
 // One output tile (T, tau pass `chunk`) of a migration: the whole forward
-// algorithm described in migrate.hip. Shared by the per-plan kernel and the
+// algorithm described in migrate.hip. GN: gained traces (sample n carries
+// sqrt(n dt)), accumulated without the per-update sqrt(t) weight; a template
+// parameter so that each form keeps its own register allocation. Shared by the per-plan kernel and the
 // persistent batch kernel; the caller provides the LDS arrays.
-template <int NTH, int TB, int TL, bool V3, bool KH>
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN>
 __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const int chunk,
                                          const int split,
                                          float2 *__restrict__ s_pairs,
@@ -278,6 +281,8 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                         const float t = acc[b][j] + y;
                         c = (t - acc[b][j]) - y;
                         acc[b][j] = t;
+                    } else if (GN) {
+                        acc[b][j] = acc[b][j] + d;
                     } else {
                         acc[b][j] = fmaf(d, __builtin_amdgcn_sqrtf(x), acc[b][j]);
                     }
@@ -291,7 +296,7 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
         stages(std::false_type{});
     else
         stages(std::true_type{});
-    // ---- epilogue: weight tau*sqrt(dt), one write per sample ---------------
+    // ---- epilogue: weight tau*sqrt(dt) (tau for gained traces), one write ---
 #pragma unroll
     for (int b = 0; b < TB; ++b) {
         const int bi = b / a.tile_y, bj = b % a.tile_y;
@@ -387,9 +392,12 @@ struct FwdLaunch {
 // device's occupancy rules. The kernel hides the latency of its LDS round
 // trips and trace loads with other resident waves, so occupancy decides:
 //  * shape: among the shapes that cover nt in one pass and fit the register
-//    budget, most resident waves per CU, then fewest idle lanes (for 8-bin
-//    tiles with a 3-D velocity, 512 lanes x 3 samples at 79 VGPRs keep three
-//    workgroups per CU where 256 x 6 keeps one);
+//    budget, most resident waves per CU, then fewest idle lanes, then more
+//    samples per lane (each (trace, bin) costs a distance-term read and a
+//    mask test, amortised over the lane's samples: 256 x 4 beat 512 x 2 by
+//    11% on the batched gathers). For 8-bin tiles with a 3-D velocity,
+//    512 lanes x 3 samples at 79 VGPRs keep three workgroups per CU where
+//    256 x 6 keeps one;
 //  * G (traces per stage): the largest of 5 .. 1 whose pair table keeps the
 //    register-limited workgroups per CU (measured on the 3-D workload: 5 for
 //    16-bin tiles at two workgroups per CU, 4 for 8-bin tiles at three);
@@ -413,7 +421,8 @@ inline psktm_status choose_launch(int nt, int ns, int tb, bool v3, bool kh, int 
             PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, nth, slot));
             const long long waves = (long long)blocks * (nth / 64);
             const long long waste = (long long)nth * tl - nt;
-            if (waves > best_waves || (waves == best_waves && waste < best_waste)) {
+            if (waves > best_waves || (waves == best_waves && (waste < best_waste ||
+                                       (waste == best_waste && tl > best.tl)))) {
                 best = {nth, tl, 1};
                 best_fn = fn;
                 best_waves = waves;

@@ -60,7 +60,7 @@ struct AdjArgs {
     float *traces;
 };
 
-template <int NTH, int TB, int TL, bool V3, bool KH>
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN>
 __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 {
     extern __shared__ __align__(16) float2 s_pairs[];
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
     // block -> (tile in work order, tau pass, split); splits of a tile adjacent
     const int split = blockIdx.x % a.n_split;
     const int rest = blockIdx.x / a.n_split;
-    fwd_tile<NTH, TB, TL, V3, KH>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks, split,
+    fwd_tile<NTH, TB, TL, V3, KH, GN>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks, split,
                                   s_pairs, s_ab, s_mask);
 }
 
@@ -254,39 +254,47 @@ __global__ void reduce_partials_kernel(const float *__restrict__ partial, int n_
 using FwdFn = void (*)(FwdArgs);
 using AdjFn = void (*)(AdjArgs);
 
-template <int NTH, int TB, int TL, bool V3, bool KH>
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN>
 FwdFn fwd_pick()
 {
-    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH))
-        return migrate_fwd_kernel<NTH, TB, TL, V3, KH>;
+    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH) && !(KH && GN))
+        return migrate_fwd_kernel<NTH, TB, TL, V3, KH, GN>;
     else
         return nullptr;
 }
 
-template <int NTH, int TB, bool V3, bool KH>
+template <int NTH, int TB, bool V3, bool KH, bool GN>
 FwdFn fwd_by_tl(int tl)
 {
     switch (tl) {
-    case 2: return fwd_pick<NTH, TB, 2, V3, KH>();
-    case 3: return fwd_pick<NTH, TB, 3, V3, KH>();
-    case 4: return fwd_pick<NTH, TB, 4, V3, KH>();
-    case 6: return fwd_pick<NTH, TB, 6, V3, KH>();
-    case 8: return fwd_pick<NTH, TB, 8, V3, KH>();
+    case 2: return fwd_pick<NTH, TB, 2, V3, KH, GN>();
+    case 3: return fwd_pick<NTH, TB, 3, V3, KH, GN>();
+    case 4: return fwd_pick<NTH, TB, 4, V3, KH, GN>();
+    case 6: return fwd_pick<NTH, TB, 6, V3, KH, GN>();
+    case 8: return fwd_pick<NTH, TB, 8, V3, KH, GN>();
     default: return nullptr;
     }
 }
 
+// mode: 0 raw traces, 1 precise (compensated), 2 gained traces
 template <int NTH, int TB>
-FwdFn fwd_by_flags(int tl, bool v3, bool kh)
+FwdFn fwd_by_flags(int tl, bool v3, int mode)
 {
-    if (v3) return kh ? fwd_by_tl<NTH, TB, true, true>(tl) : fwd_by_tl<NTH, TB, true, false>(tl);
-    return kh ? fwd_by_tl<NTH, TB, false, true>(tl) : fwd_by_tl<NTH, TB, false, false>(tl);
+    if (v3) {
+        if (mode == 1) return fwd_by_tl<NTH, TB, true, true, false>(tl);
+        if (mode == 2) return fwd_by_tl<NTH, TB, true, false, true>(tl);
+        return fwd_by_tl<NTH, TB, true, false, false>(tl);
+    }
+    if (mode == 1) return fwd_by_tl<NTH, TB, false, true, false>(tl);
+    if (mode == 2) return fwd_by_tl<NTH, TB, false, false, true>(tl);
+    return fwd_by_tl<NTH, TB, false, false, false>(tl);
 }
 
-FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh)
+FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh, bool gn)
 {
-    if (nth == 256) return tb == 8 ? fwd_by_flags<256, 8>(tl, v3, kh) : fwd_by_flags<256, 16>(tl, v3, kh);
-    return tb == 8 ? fwd_by_flags<512, 8>(tl, v3, kh) : fwd_by_flags<512, 16>(tl, v3, kh);
+    const int mode = kh ? 1 : (gn ? 2 : 0);
+    if (nth == 256) return tb == 8 ? fwd_by_flags<256, 8>(tl, v3, mode) : fwd_by_flags<256, 16>(tl, v3, mode);
+    return tb == 8 ? fwd_by_flags<512, 8>(tl, v3, mode) : fwd_by_flags<512, 16>(tl, v3, mode);
 }
 
 } // namespace
@@ -333,14 +341,18 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     PSKTM_CHECK(p->n_traces == 0 || tr.data, PSKTM_E_INVAL, "trace data is NULL");
     PSKTM_CHECK(tr.dtype == PSKTM_F32 || p->n_traces == 0 || tr.scale, PSKTM_E_INVAL,
                 "16-bit traces need per-trace scales");
+    PSKTM_CHECK(tr.gained == 0 || tr.gained == 1, PSKTM_E_INVAL, "gained must be 0 or 1");
+    PSKTM_CHECK(!(tr.gained && p->opt.precise), PSKTM_E_UNSUPPORTED,
+                "precise mode takes raw traces (gained = 0)");
     const psktm_grid &g = p->grid;
     const int tb = p->tile_x * p->tile_y;
     const bool v3 = vel.kind == PSKTM_VEL_3D;
     const bool kh = p->opt.precise != 0;
+    const bool gn = tr.gained != 0;
     FwdLaunch L{};
     if ((s = choose_launch(g.nt, p->n_samples, tb, v3, kh, p->opt.traces_per_stage,
                            p->opt.row_pad,
-                           [&](int nth, int tl) { return (const void *)fwd_kernel(nth, tb, tl, v3, kh); },
+                           [&](int nth, int tl) { return (const void *)fwd_kernel(nth, tb, tl, v3, kh, gn); },
                            &L)) != PSKTM_OK)
         return s;
     const FwdShape sh = L.shape;
@@ -352,6 +364,8 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     fwd_args_from_plan(p, a);
     a.n_chunks = sh.chunks;
     a.dtype = tr.dtype;
+    a.gained = tr.gained;
+    if (tr.gained) a.wscale = 1.0f;
     a.G = G;
     a.nsp = nsp;
     a.lim = p->opt.row_pad < 0 ? 0 : pair_limit(nsp);

@@ -16,7 +16,9 @@
 //
 // Outputs: FP32, or FP16/BF16 with a per-trace scale s_k = max|y_k| / q
 // (q = 32768 for FP16, keeping values well inside its range; q = 1 for BF16),
-// stored value = y / s_k rounded to nearest even.
+// stored value = y / s_k rounded to nearest even. With a gain, y at sample n
+// is first multiplied by sqrtf(n * gain_dt): the sqrt(t) part of the
+// migration weight, applied once per trace sample instead of once per update.
 #include "internal.hpp"
 
 #include <hip/hip_fp16.h>
@@ -87,10 +89,20 @@ __device__ __forceinline__ void store_q(void *out, int dtype, size_t idx, float 
     else static_cast<unsigned short *>(out)[idx] = f2bf16_rne(y * inv);
 }
 
+// sqrtf(n * gain_dt), or 1 without a gain (gain_dt == 0); one rounding for the
+// product, correctly rounded sqrt.
+__device__ __forceinline__ float row_gain(int n, float gain_dt)
+{
+#pragma clang fp contract(off)
+    if (gain_dt == 0.0f) return 1.0f;
+    const float t = (float)n * gain_dt;
+    return sqrtf(t);
+}
+
 __global__ __launch_bounds__(kRhoThreads) void rho_kernel(
     const float *__restrict__ in, int n, int m, int M, int logM, float wscale, int opt,
     int dtype, float qmax, void *__restrict__ out, float *__restrict__ scale_out,
-    int shift, const float2 *__restrict__ twg)
+    int shift, const float2 *__restrict__ twg, float gain_dt)
 {
     extern __shared__ float2 smem[];
     __shared__ float s_red[kRhoThreads / 64];
@@ -138,6 +150,17 @@ __global__ __launch_bounds__(kRhoThreads) void rho_kernel(
     float2 *buf_a = other, *buf_b = Z;
     float2 *Y = stockham(buf_a, buf_b, tw, M, logM, true);
     const float invM = 1.0f / (float)M;
+    if (gain_dt != 0.0f) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            const float gq = row_gain(q, gain_dt);
+            const float ya = Y[q].x * invM, yb = Y[q].y * invM;
+            // stored back pre-multiplied by M, so the common path below
+            // divides by M exactly as without a gain (M is a power of two)
+            Y[q] = make_float2((ya * gq) * (float)M, (yb * gq) * (float)M);
+        }
+        __syncthreads();
+    }
     float inv_a = 1.f, inv_b = 1.f;
     if (dtype != PSKTM_F32) {
         float ma = 0.f, mb = 0.f;
@@ -161,19 +184,28 @@ __global__ __launch_bounds__(kRhoThreads) void rho_kernel(
     }
 }
 
+// One trace per workgroup: optional gain, then F32 out or per-trace max-abs
+// quantisation.
 __global__ __launch_bounds__(kRhoThreads) void quantize_kernel(
     const float *__restrict__ in, int m, int dtype, float qmax, void *__restrict__ out,
-    float *__restrict__ scale_out)
+    float *__restrict__ scale_out, float gain_dt)
 {
+#pragma clang fp contract(off)
     __shared__ float s_red[kRhoThreads / 64];
     const size_t row = (size_t)blockIdx.x * m;
+    auto value = [&](int q) { return in[row + q] * row_gain(q, gain_dt); };
+    if (dtype == PSKTM_F32) {
+        for (int q = threadIdx.x; q < m; q += blockDim.x)
+            static_cast<float *>(out)[row + q] = value(q);
+        return;
+    }
     float mx = 0.f;
-    for (int q = threadIdx.x; q < m; q += blockDim.x) mx = fmaxf(mx, fabsf(in[row + q]));
+    for (int q = threadIdx.x; q < m; q += blockDim.x) mx = fmaxf(mx, fabsf(value(q)));
     mx = block_max(mx, s_red);
     const float s = mx / qmax;
     const float inv = (s > 0.f) ? 1.0f / s : 0.f;
     if (threadIdx.x == 0) scale_out[blockIdx.x] = s;
-    for (int q = threadIdx.x; q < m; q += blockDim.x) store_q(out, dtype, row + q, in[row + q], inv);
+    for (int q = threadIdx.x; q < m; q += blockDim.x) store_q(out, dtype, row + q, value(q), inv);
 }
 
 __global__ void ibm_kernel(const unsigned *__restrict__ in, float *__restrict__ out, size_t n)
@@ -230,7 +262,17 @@ psktm_status psktm_rho_filter(const float *in, int n_traces, int n_samples, floa
                               int opt, int out_dtype, void *out, float *scale_out,
                               int matlab_trace_shift, void *stream)
 {
+    return psktm_rho_filter_gain(in, n_traces, n_samples, dw_us, opt, out_dtype, out, scale_out,
+                                 matlab_trace_shift, 0.0f, stream);
+}
+
+psktm_status psktm_rho_filter_gain(const float *in, int n_traces, int n_samples, float dw_us,
+                                   int opt, int out_dtype, void *out, float *scale_out,
+                                   int matlab_trace_shift, float gain_dt, void *stream)
+{
     psktm__clear_error();
+    PSKTM_CHECK(std::isfinite(gain_dt) && gain_dt >= 0.0f, PSKTM_E_INVAL,
+                "gain_dt must be finite and non-negative");
     PSKTM_CHECK(n_traces >= 0 && n_samples > 0, PSKTM_E_INVAL, "bad sizes (%d x %d)", n_traces,
                 n_samples);
     PSKTM_CHECK(opt == 1 || opt == 2 || opt == 3, PSKTM_E_INVAL, "rho option must be 1, 2 or 3");
@@ -260,8 +302,28 @@ psktm_status psktm_rho_filter(const float *in, int n_traces, int n_samples, floa
     const unsigned blocks = (unsigned)((n_traces + 1) / 2);
     hipLaunchKernelGGL(rho_kernel, dim3(blocks), dim3(kRhoThreads), lds, as_stream(stream), in,
                        n_traces, n_samples, M, logM, wscale, opt, out_dtype,
-                       qmax_of(out_dtype), out, scale_out, matlab_trace_shift ? 1 : 0, tw);
+                       qmax_of(out_dtype), out, scale_out, matlab_trace_shift ? 1 : 0, tw,
+                       gain_dt);
     return check_launch("rho_kernel");
+}
+
+psktm_status psktm_trace_gain(const float *in, int n_traces, int n_samples, float gain_dt,
+                              int out_dtype, void *out, float *scale_out, void *stream)
+{
+    psktm__clear_error();
+    PSKTM_CHECK(n_traces >= 0 && n_samples > 0, PSKTM_E_INVAL, "bad sizes");
+    PSKTM_CHECK(std::isfinite(gain_dt) && gain_dt > 0.0f, PSKTM_E_INVAL,
+                "gain_dt must be positive");
+    PSKTM_CHECK(out_dtype == PSKTM_F32 || out_dtype == PSKTM_F16 || out_dtype == PSKTM_BF16,
+                PSKTM_E_INVAL, "unknown output dtype %d", out_dtype);
+    if (n_traces == 0) return PSKTM_OK;
+    PSKTM_CHECK(in && out && (out_dtype == PSKTM_F32 || scale_out), PSKTM_E_INVAL,
+                "NULL pointer");
+    psktm_status st;
+    if ((st = require_device()) != PSKTM_OK) return st;
+    hipLaunchKernelGGL(quantize_kernel, dim3(n_traces), dim3(kRhoThreads), 0, as_stream(stream),
+                       in, n_samples, out_dtype, qmax_of(out_dtype), out, scale_out, gain_dt);
+    return check_launch("quantize_kernel");
 }
 
 psktm_status psktm_quantize(const float *in, int n_traces, int n_samples, int out_dtype,
@@ -276,7 +338,7 @@ psktm_status psktm_quantize(const float *in, int n_traces, int n_samples, int ou
     psktm_status st;
     if ((st = require_device()) != PSKTM_OK) return st;
     hipLaunchKernelGGL(quantize_kernel, dim3(n_traces), dim3(kRhoThreads), 0, as_stream(stream),
-                       in, n_samples, out_dtype, qmax_of(out_dtype), out, scale_out);
+                       in, n_samples, out_dtype, qmax_of(out_dtype), out, scale_out, 0.0f);
     return check_launch("quantize_kernel");
 }
 

@@ -11,7 +11,7 @@ from ._lib import (BF16, F16, F32, INTERP_LINEAR, INTERP_TRUNC, MATH_FAST, MATH_
 from .api import (Batch, Grid, Plan, Survey, demigrate, dot_f64, ibm32_to_ieee,
                   ibm32_to_ieee_device, ibm32_to_ieee_exact, init_three_axes,
                   init_two_axes, ktm_migration_gpu, load_ktm_config, migrate,
-                  migrate_exact, quantize, rho_filter, offset_sort, offset_classes,
+                  migrate_exact, quantize, rho_filter, trace_gain, offset_sort, offset_classes,
                   select_offset, gather_pad, upsample_traces, segy_text_header,
                   segy_reel_header, segy_open, segy_trace_headers, segy_decode, segy_load)
 
@@ -21,7 +21,7 @@ __all__ = [
     "Batch", "Grid", "Plan", "Survey", "demigrate", "dot_f64", "ibm32_to_ieee",
     "ibm32_to_ieee_device", "ibm32_to_ieee_exact", "init_three_axes", "init_two_axes",
     "ktm_migration_gpu", "load_ktm_config", "migrate", "migrate_exact", "quantize",
-    "rho_filter", "offset_sort", "offset_classes", "select_offset", "gather_pad",
+    "rho_filter", "trace_gain", "offset_sort", "offset_classes", "select_offset", "gather_pad",
     "upsample_traces", "segy_text_header", "segy_reel_header", "segy_open",
     "segy_trace_headers", "segy_decode", "segy_load",
 ]

@@ -51,7 +51,7 @@ class PsktmSurvey(ctypes.Structure):
 
 
 class PsktmTraces(ctypes.Structure):
-    _fields_ = [("dtype", _c_int), ("data", _p), ("scale", _p)]
+    _fields_ = [("dtype", _c_int), ("data", _p), ("scale", _p), ("gained", _c_int)]
 
 
 class PsktmVelocity(ctypes.Structure):
@@ -110,6 +110,9 @@ _SIGNATURES = {
     "psktm_batch_migrate": (_c_int, [_p, _P(PsktmTraces), _P(PsktmVelocity), _p, _c_int, _p]),
     "psktm_migrate_exact": (_c_int, [_P(PsktmGrid), _P(PsktmSurvey), _p, _p, _p, _p]),
     "psktm_rho_filter": (_c_int, [_p, _c_int, _c_int, _c_float, _c_int, _c_int, _p, _p, _c_int, _p]),
+    "psktm_rho_filter_gain": (_c_int, [_p, _c_int, _c_int, _c_float, _c_int, _c_int, _p, _p, _c_int,
+                                       _c_float, _p]),
+    "psktm_trace_gain": (_c_int, [_p, _c_int, _c_int, _c_float, _c_int, _p, _p, _p]),
     "psktm_quantize": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p, _p]),
     "psktm_ibm32_to_ieee_exact": (None, [_p, _p, _c_int]),
     "psktm_ibm32_to_ieee_device": (_c_int, [_p, _p, _c_size, _p]),

@@ -197,7 +197,8 @@ class Batch:
         check(_lib.load().psktm_batch_updates(self._h, ctypes.byref(v)))
         return v.value
 
-    def migrate(self, traces, vel, images=None, *, scale=None, use_graph=False, stream=None):
+    def migrate(self, traces, vel, images=None, *, scale=None, use_graph=False, gained=False,
+                stream=None):
         torch = _torch()
         L = _lib.load()
         kinds = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
@@ -210,7 +211,8 @@ class Batch:
         if images is None:
             images = torch.empty(total, dtype=torch.float32, device=traces.device)
         _need(images, torch.float32, "images", total)
-        tr = PsktmTraces(kind, _ptr(traces), _ptr(scale) if kind != _lib.F32 else 0)
+        tr = PsktmTraces(kind, _ptr(traces), _ptr(scale) if kind != _lib.F32 else 0,
+                         int(bool(gained)))
         v = PsktmVelocity(VEL_1D, _ptr(vel))
         check(L.psktm_batch_migrate(self._h, ctypes.byref(tr), ctypes.byref(v), _p(_ptr(images)),
                                     int(bool(use_graph)), _stream(stream)))
@@ -235,11 +237,13 @@ class Batch:
 
 
 def migrate(plan: Plan, traces, vel, image=None, *, scale=None, dtype=None, accumulate=False,
-            stream=None):
+            gained=False, stream=None):
     """Migration traces -> image on the plan's grid (device tensors).
 
     ``traces`` is float32, float16 or bfloat16 [n_traces, n_samples]; for the
     16-bit forms ``scale`` [n_traces] float32 gives the per-trace scale.
+    ``gained``: the samples already carry sqrt(n dt) (:func:`rho_filter` with
+    ``gain=True`` or :func:`trace_gain`), so the kernel applies tau only.
     """
     torch = _torch()
     L = _lib.load()
@@ -251,7 +255,8 @@ def migrate(plan: Plan, traces, vel, image=None, *, scale=None, dtype=None, accu
     kind = kinds[traces.dtype]
     if kind != _lib.F32:
         _need(scale, torch.float32, "scale", s.n_traces)
-    tr = PsktmTraces(kind, _ptr(traces), _ptr(scale) if kind != _lib.F32 else 0)
+    tr = PsktmTraces(kind, _ptr(traces), _ptr(scale) if kind != _lib.F32 else 0,
+                     int(bool(gained)))
     v = _velocity(vel, g)
     if image is None:
         image = torch.empty(g.shape, dtype=torch.float32, device=traces.device)
@@ -277,11 +282,13 @@ def demigrate(plan: Plan, image, vel, traces=None, *, accumulate=False, stream=N
 
 
 def rho_filter(traces, dw_us: float, opt: int = 2, out_dtype=None, *, matlab_trace_shift=False,
-               stream=None):
+               gain=False, stream=None):
     """Rho filter (scripts/ktmApplyRhoFilter.m) on device traces [n, m].
 
     Returns float32 traces, or (quantised traces, per-trace scale) when
-    ``out_dtype`` is torch.float16 / torch.bfloat16.
+    ``out_dtype`` is torch.float16 / torch.bfloat16. ``gain=True`` multiplies
+    sample n by sqrt(n dt) (dt = dw_us 1e-6 s) before quantising: migrate the
+    result with ``gained=True``.
     """
     torch = _torch()
     L = _lib.load()
@@ -292,9 +299,27 @@ def rho_filter(traces, dw_us: float, opt: int = 2, out_dtype=None, *, matlab_tra
     out = torch.empty((n, m), dtype=out_dtype, device=traces.device)
     scale = torch.empty((n,), dtype=torch.float32, device=traces.device) \
         if out_dtype != torch.float32 else None
-    check(L.psktm_rho_filter(_p(_ptr(traces)), int(n), int(m), float(dw_us), int(opt),
-                             kinds[out_dtype], _p(_ptr(out)), _p(_ptr(scale)),
-                             int(bool(matlab_trace_shift)), _stream(stream)))
+    gain_dt = float(dw_us) * 1e-6 if gain else 0.0
+    check(L.psktm_rho_filter_gain(_p(_ptr(traces)), int(n), int(m), float(dw_us), int(opt),
+                                  kinds[out_dtype], _p(_ptr(out)), _p(_ptr(scale)),
+                                  int(bool(matlab_trace_shift)), gain_dt, _stream(stream)))
+    return out if scale is None else (out, scale)
+
+
+def trace_gain(traces, dt: float, out_dtype=None, *, stream=None):
+    """sample n times sqrtf(n dt) (the trace-domain part of the weight), as
+    float32, or (quantised traces, per-trace scale) for float16 / bfloat16."""
+    torch = _torch()
+    L = _lib.load()
+    _need(traces, torch.float32, "traces")
+    n, m = traces.shape
+    out_dtype = torch.float32 if out_dtype is None else out_dtype
+    kinds = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
+    out = torch.empty((n, m), dtype=out_dtype, device=traces.device)
+    scale = torch.empty((n,), dtype=torch.float32, device=traces.device) \
+        if out_dtype != torch.float32 else None
+    check(L.psktm_trace_gain(_p(_ptr(traces)), int(n), int(m), float(dt), kinds[out_dtype],
+                             _p(_ptr(out)), _p(_ptr(scale)), _stream(stream)))
     return out if scale is None else (out, scale)
 
 

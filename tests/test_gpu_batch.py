@@ -60,6 +60,31 @@ def test_batch_row_padding_is_bit_identical(cuda):
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
 
 
+def test_batch_gained_equals_per_gather_plans(cuda):
+    """Gained traces in the persistent launch (and its graph replay) equal
+    one-workgroup-per-tile plans migrating the same gained traces."""
+    import torch
+    gd = small_gathers(cuda, n=6, seed=34)
+    q, sc = psktm.rho_filter(gd.traces, gd.dt * 1e6, 2, out_dtype=torch.float16, gain=True)
+    batch, _ = make_batch(gd, cuda)
+    imgs = batch.migrate(q, gd.vel, scale=sc, gained=True).clone()
+    rep = batch.migrate(q, gd.vel, scale=sc, gained=True, use_graph=True)
+    rep = batch.migrate(q, gd.vel, scale=sc, gained=True, use_graph=True).clone()
+    raw = batch.migrate(q, gd.vel, scale=sc).clone()
+    torch.cuda.synchronize()
+    assert torch.equal(imgs, rep)
+    assert not torch.equal(imgs, raw)
+    for g, (grid, f, c) in enumerate(zip(gd.grids, gd.first, gd.counts)):
+        sl = slice(f, f + c)
+        s = psktm.Survey(gd.mid_x[sl].contiguous(), gd.mid_y[sl].contiguous(),
+                         gd.off_x[sl].contiguous(), gd.off_y[sl].contiguous(),
+                         gd.traces.shape[1], gd.dt)
+        plan = psktm.Plan(psktm.Grid(*grid), s, aperture=gd.aperture, splits=1)
+        one = psktm.migrate(plan, q[sl].contiguous(), gd.vel, scale=sc[sl].contiguous(),
+                            gained=True)
+        assert torch.equal(batch.image(imgs, g), one), g
+
+
 def test_batch_vs_fp64_oracle(cuda):
     gd = small_gathers(cuda, n=3, scale=0.05, seed=32)
     batch, _ = make_batch(gd, cuda)

@@ -282,3 +282,78 @@ def test_row_padding_is_bit_identical(cuda, case):
         if interp == psktm.INTERP_LINEAR:
             ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
             assert rel_l2(outs[0][0], ref) <= 1e-4
+
+
+def _gain_np(x, dt):
+    """sample n times sqrtf(n dt) in binary32 (one rounding per operation)."""
+    n = np.arange(x.shape[1], dtype=np.float32)
+    g = np.sqrt(n * np.float32(dt)).astype(np.float32)
+    return (x.astype(np.float32) * g[None]).astype(np.float32)
+
+
+def test_trace_gain_and_rho_gain_bit_exact(cuda):
+    """psktm_trace_gain is in[k][n] * sqrtf(n dt) in binary32; the rho filter's
+    fused gain equals the ungained filter output times the same factors, and its
+    FP16 form equals quantising the FP32 gained output."""
+    import torch
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal((7, 333)).astype(np.float32)
+    dt = 0.004
+    xt = torch.as_tensor(x).to(cuda)
+    got = psktm.trace_gain(xt, dt).cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), _gain_np(x, dt).view(np.uint32))
+    plain = psktm.rho_filter(xt, dt * 1e6, 2).cpu().numpy()
+    gained = psktm.rho_filter(xt, dt * 1e6, 2, gain=True)
+    assert np.array_equal(gained.cpu().numpy().view(np.uint32),
+                          _gain_np(plain, np.float32(dt * 1e6) * 1e-6).view(np.uint32)) or \
+        np.array_equal(gained.cpu().numpy().view(np.uint32), _gain_np(plain, dt).view(np.uint32))
+    for dtype in (torch.float16, torch.bfloat16):
+        q1, s1 = psktm.rho_filter(xt, dt * 1e6, 2, out_dtype=dtype, gain=True)
+        q2, s2 = psktm.quantize(gained, dtype)
+        assert torch.equal(q1, q2) and torch.equal(s1, s2), dtype
+        q3, s3 = psktm.trace_gain(torch.as_tensor(plain).to(cuda), dt, dtype)
+        assert torch.equal(s3, s2) or float((s3 - s2).abs().max()) <= 1e-6 * float(s2.abs().max())
+
+
+@pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
+def test_gained_traces_vs_fp64(cuda, interp):
+    """Gained traces migrated with tau alone: the GPU form against the FP64
+    oracle's gained form on exactly the values the kernel reads (1e-4, as the
+    per-update form), and - LINEAR - against the FP64 per-update weight on the
+    raw traces, where the difference is the interpolation of the weight
+    (bounded by the metric's 1e-3)."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=43, ntr=2000, nx=9, ny=7)
+    ns = tr.shape[1]
+    plan, t = gpu_plan(cuda, grid, mid, off, ns, dt, ap, interp=interp)
+    g = psktm.trace_gain(t(tr), dt)
+    img = psktm.migrate(plan, g, t(vel), gained=True).cpu().numpy()
+    oi = oracle.LINEAR if interp == psktm.INTERP_LINEAR else oracle.TRUNC
+    ref_g = oracle.pstm_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1],
+                            g.cpu().numpy(), vel, dt, vel3d=True, aperture=ap, interp=oi,
+                            gained=True)
+    e = rel_l2(img, ref_g)
+    print(f"gained interp={interp}: rel-L2 vs FP64 gained form {e:.3e}")
+    assert e <= (1e-4 if interp == psktm.INTERP_LINEAR else 2e-2)
+    if interp == psktm.INTERP_LINEAR:
+        ref_w = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+        ew = rel_l2(img, ref_w)
+        print(f"gained vs per-update weight (FP64): {ew:.3e}")
+        assert ew <= 1e-3
+        # FP16 through the rho filter's fused gain: the headline form
+        dw = dt * 1e6
+        q, sc = psktm.rho_filter(t(tr), dw, 2, out_dtype=torch.float16, gain=True)
+        img16 = psktm.migrate(plan, q, t(vel), scale=sc, gained=True).cpu().numpy()
+        rho = psktm.rho_filter(t(tr), dw, 2).cpu().numpy()
+        ref16 = oracle_f64(grid, mid, off, rho, vel, dt, ap, oracle.LINEAR)
+        e16 = rel_l2(img16, ref16)
+        print(f"FP16 gained (rho) vs FP64 per-update weight: {e16:.3e}")
+        assert e16 <= 1e-3
+
+
+def test_gained_precise_unsupported_and_batch(cuda):
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=44, ntr=500, nx=5, ny=5)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, precise=True)
+    with pytest.raises(psktm.PsktmError):
+        psktm.migrate(plan, t(tr), t(vel), gained=True)

@@ -200,6 +200,60 @@ def test_fp64_linear_exact_on_linear_trace():
             assert out[i, 0, l] == pytest.approx(expect, rel=1e-11, abs=1e-12)
 
 
+def test_fp64_gained_closed_form_on_linear_trace():
+    """Gained traces (sample n carries sqrt(n dt)) are migrated with weight tau
+    alone: on a trace that is linear in n the interpolation is exact, so the
+    image is tau * g(t/dt), from source/receiver positions in FP64. A weight
+    that kept sqrt(t), or lost tau, fails here."""
+    grid = Grid(3, -20.0, 30.0, 2, 0.0, 15.0, 50, 0.002, 0.004)
+    mid = np.array([10.0, 35.0], np.float32)
+    midy = np.array([5.0, -12.0], np.float32)
+    offx = np.array([120.0, -60.0], np.float32)
+    offy = np.array([40.0, 90.0], np.float32)
+    ns, dt = 180, DT
+    g = np.stack([1.5 + 0.25 * np.arange(ns), -2.0 + 0.125 * np.arange(ns)]).astype(np.float32)
+    vel = np.linspace(1800, 2600, 50).astype(np.float32)
+    out = oracle.pstm_f64(grid, mid, midy, offx, offy, g, vel, dt, interp=oracle.LINEAR,
+                          gained=True)
+    _, _, zz = grid.axes()
+    ref = np.zeros(out.shape)
+    for k in range(2):
+        src = np.array([mid[k] - offx[k], midy[k] - offy[k]], np.float64)
+        rec = np.array([mid[k] + offx[k], midy[k] + offy[k]], np.float64)
+        for i in range(3):
+            for j in range(2):
+                for l in range(50):
+                    q = brute_dsr_f64(grid, src, rec, l, (i, j), vel[l]) / dt
+                    if q < ns - 1:
+                        ref[i, j, l] += float(zz[l]) * (float(g[k, 0]) + float(g[k, 1] - g[k, 0]) * q)
+    assert np.any(ref != 0)
+    assert np.allclose(out, ref, rtol=1e-11, atol=1e-12)
+
+
+def test_fp64_gained_equals_weighted_where_the_gain_is_exact():
+    """TRUNC reads one sample at n = floor(t/dt): the per-update weight
+    tau*sqrt(t) and the gained form tau*d[n]*sqrt(n dt) differ only by
+    sqrt(t) versus sqrt(n dt). With d[n] = c/sqrt(n dt) the gained trace is
+    the constant c, so gained = sum tau c, while the weighted form gives
+    sum tau c sqrt(t/(n dt)) >= the gained value, within the factor
+    sqrt((n+1)/n) of it."""
+    grid, src, rec, mid, off, vel, _, dt = tiny_case(9, ntr=3, nx=3, ny=2, nt=40, ns=300)
+    n = np.arange(300, dtype=np.float64)
+    d = np.zeros((3, 300), np.float32)
+    d[:, 1:] = (2.0 / np.sqrt(n[1:] * dt)).astype(np.float32)
+    gtr = (d.astype(np.float64) * np.sqrt(n * dt)).astype(np.float32)
+    w = oracle.pstm_f64(grid, mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], d, vel, dt,
+                        interp=oracle.TRUNC)
+    gn = oracle.pstm_f64(grid, mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], gtr, vel, dt,
+                         interp=oracle.TRUNC, gained=True)
+    _, _, zz = grid.axes()
+    assert np.all(gn >= 0) and np.any(gn > 0)
+    nz = gn > 0
+    ratio = w[nz] / gn[nz]
+    assert np.all(ratio >= 1 - 1e-6)
+    assert np.all(ratio <= math.sqrt(2.0) + 1e-6)
+
+
 @pytest.mark.parametrize("interp", [oracle.TRUNC, oracle.LINEAR])
 def test_fp64_adjoint_dot_product(interp):
     rng = np.random.default_rng(11)
