@@ -359,9 +359,10 @@ def main():
 def bench_batched(args):
     """64 offset-class gathers (lognormal 2,000-60,000 traces, patches of
     32-128 bins at 25 m x 1,000 tau, 1,000 samples at 4 ms, aperture 1500 m),
-    FP16 traces + FP32 scales, migrated by ONE persistent launch replayed from
-    a HIP graph, tile shape from the autotuner; compared with launching the
-    same gathers one plan at a time."""
+    FP16 traces + FP32 scales, gained by sqrt(n dt) in the rho filter (as the
+    headline form), migrated by ONE persistent launch replayed from a HIP
+    graph, tile shape from the autotuner; compared with launching the same
+    gathers one plan at a time."""
     import numpy as np
     import torch
 
@@ -375,20 +376,20 @@ def bench_batched(args):
     torch.cuda.synchronize()
     log(f"64 gathers, {sum(gd.counts)} traces x {gd.traces.shape[1]} in {time.time() - t0:.1f} s")
     ns = gd.traces.shape[1]
-    q, sc = psktm.rho_filter(gd.traces, gd.dt * 1e6, 2, out_dtype=torch.float16)
+    q, sc = psktm.rho_filter(gd.traces, gd.dt * 1e6, 2, out_dtype=torch.float16, gain=True)
     survey = psktm.Survey(gd.mid_x, gd.mid_y, gd.off_x, gd.off_y, ns, gd.dt)
     gathers = [(psktm.Grid(*g), f, c) for g, f, c in zip(gd.grids, gd.first, gd.counts)]
 
     # autotune on every fourth gather (a quarter of the work)
     sub = gathers[::4]
-    key = signature("batch", ns=ns, nt=gd.grids[0][6], dtype="f16", aperture=gd.aperture,
-                    gathers=len(gathers), seed=gd.seed)
+    key = signature("batch", ns=ns, nt=gd.grids[0][6], dtype="f16", gained=True,
+                    aperture=gd.aperture, gathers=len(gathers), seed=gd.seed)
     tuner = Autotuner(args.tune_cache)
 
     def trial(c):
         b = psktm.Batch(sub, survey, aperture=gd.aperture, tile_x=c[0], tile_y=c[1],
                         traces_per_stage=c[2])
-        t = time_gpu(lambda: b.migrate(q, gd.vel, scale=sc), repeats=1)
+        t = time_gpu(lambda: b.migrate(q, gd.vel, scale=sc, gained=True), repeats=1)
         log(f"  tune {c}: {t * 1e3:.1f} ms")
         b.close()
         return t
@@ -401,11 +402,11 @@ def bench_batched(args):
     updates = batch.updates()
     images = torch.empty(int(sum(batch.sizes)), dtype=torch.float32, device=dev)
     for _ in range(max(1, args.warmup)):
-        batch.migrate(q, gd.vel, images, scale=sc, use_graph=True)
+        batch.migrate(q, gd.vel, images, scale=sc, use_graph=True, gained=True)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     for _ in range(args.steps):
-        batch.migrate(q, gd.vel, images, scale=sc, use_graph=True)
+        batch.migrate(q, gd.vel, images, scale=sc, use_graph=True, gained=True)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t2) / args.steps
     # the same gathers launched one plan at a time: with one workgroup per
@@ -424,7 +425,7 @@ def bench_batched(args):
 
         def sequential():
             for (p, sl), img in zip(plans, imgs):
-                psktm.migrate(p, q[sl], gd.vel, img, scale=sc[sl])
+                psktm.migrate(p, q[sl], gd.vel, img, scale=sc[sl], gained=True)
 
         sequential()
         torch.cuda.synchronize()
@@ -446,7 +447,8 @@ def bench_batched(args):
         "data": "synthetic",
         "config": {"workload": "64 offset-class gathers, lognormal(ln 11000, 0.8) traces clipped "
                                "to 2,000-60,000, patches 32-128 bins at 25 m x 1,000 tau, "
-                               "1,000 samples at 4 ms, aperture 1500 m, FP16 traces",
+                               "1,000 samples at 4 ms, aperture 1500 m, FP16 traces gained by "
+                               "sqrt(n dt)",
                    "traces": int(sum(gd.counts)), "updates_per_step": updates,
                    "tile": list(choice), "graph": True},
         "sequential_ms": seq * 1e3, "persistent_over_sequential": seq / wall,
