@@ -448,7 +448,10 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
                            traces_out, ntot);
     if (p->n_adj_items == 0) return check_launch("zero_kernel");
 
-    constexpr int NTH = 256, TL = 8;
+    // 512 lanes x 4 consecutive tau samples: 16 waves per CU at two
+    // workgroups (LDS-bound by the FP64 rows); 256 x 8 kept 8 waves per CU
+    // and took 66.3 ms on the 2-D line against 48.9 ms, 1024 x 2 74.0 ms
+    constexpr int NTH = 512, TL = 4;
     const AdjFn fn = (tb == 8)
         ? (v3 ? demigrate_kernel<NTH, 8, TL, true, double> : demigrate_kernel<NTH, 8, TL, false, double>)
         : (v3 ? demigrate_kernel<NTH, 16, TL, true, double> : demigrate_kernel<NTH, 16, TL, false, double>);
