@@ -16,15 +16,18 @@
 //               acc += d * sqrt(x)
 //             with s' = 1/(v dt)^2 and tau' = tau/dt; the weight
 //             t*sqrt(tau^2/t) = tau*sqrt(dt)*sqrt(x) (src/psktmCPU.c:84-85)
-//             is completed by one multiply per sample at the end.
-// Out-of-range samples need no test: x is clamped to ns and the pair table's
-// entries outside the valid range of the interpolation rule are zero
-// (TRUNC: 0 < n < ns, src/psktmCPU.c:82; LINEAR: 0 <= n, n+1 < ns).
-// The path is bound by VALU issue (three v_sqrt_f32 and ~9 other VALU ops
-// per update); HBM traffic is the trace rows, read once per tile.
+//             is completed by one multiply per sample at the end. Gained
+//             traces (sample n already times sqrt(n dt)) accumulate d alone
+//             (GN instantiation: one v_sqrt_f32 fewer) and end with tau.
+// Out-of-range samples need no test: the pair table's entries outside the
+// valid range of the interpolation rule are zero (TRUNC: 0 < n < ns,
+// src/psktmCPU.c:82; LINEAR: 0 <= n, n+1 < ns), and x is clamped to ns unless
+// the tile's traveltimes are bounded inside the zero-padded rows.
+// The path is bound by VALU issue (two or three v_sqrt_f32 and ~8 other VALU
+// ops per update); HBM traffic is the trace rows, read once per tile.
 //
 // Adjoint (scatter, trace-stationary). A workgroup owns a batch of traces
-// from one midpoint cell, accumulates them in LDS with ds_add_f32, visits
+// from one midpoint cell, accumulates them in LDS with ds_add_f64, visits
 // every tile within reach of the cell and writes each trace once. It uses the
 // same traveltime, clamp, interpolation weights and aperture bits as the
 // forward, so the pair is an exact adjoint up to FP32 rounding.
