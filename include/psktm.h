@@ -194,7 +194,7 @@ typedef struct psktm_options {
     int   math;               /* psktm_math */
     float aperture;           /* metres, midpoint-to-bin; <= 0 means none */
     int   tile_x, tile_y;     /* bins per tile (8 or 16 in all); 0 = library
-                                 choice: 4x2 on a 3-D grid, 16x1 on a line */
+                                 choice: 4x2 on a 3-D grid, 8x1 on a line */
     int   traces_per_stage;   /* traces staged in LDS at once (1-8); 0 = the
                                  largest up to 5 that keeps the kernel's
                                  workgroups per CU */

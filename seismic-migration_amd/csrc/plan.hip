@@ -468,14 +468,17 @@ psktm_status psktm_plan_create(psktm_plan **plan, const psktm_grid *grid,
                 "row_pad must be in [-1, 65536]");
     PSKTM_CHECK((grid->nx == 1 || grid->dx > 0.0f) && (grid->ny == 1 || grid->dy > 0.0f),
                 PSKTM_E_INVAL, "bin spacings must be positive");
-    // Default tiles: 16 bins on a line; 4x2 bins on a 3-D grid, whose forward
-    // kernel (24 FP32 accumulators and slowness samples per lane, 79 VGPRs)
-    // keeps three 512-lane workgroups per CU where 16-bin tiles keep two
-    // (3% faster on the 3-D workload with 1-D and with 3-D velocity).
+    // Default tiles of 8 bins: 4x2 on a 3-D grid, whose forward kernel (24
+    // FP32 accumulators and slowness samples per lane, 79 VGPRs) keeps three
+    // 512-lane workgroups per CU where 16-bin tiles keep two (3% faster on the
+    // 3-D workload with 1-D and with 3-D velocity); 8x1 on a line, where the
+    // precise-mode forward drops from 165 to 108 VGPRs (two workgroups per CU
+    // instead of one: 12% faster on the least-squares line, whose adjoint
+    // walks the smaller tiles 2% slower).
     int tx = opt->tile_x, ty = opt->tile_y;
     if (tx <= 0 || ty <= 0) {
-        if (grid->ny == 1) { tx = 16; ty = 1; }
-        else if (grid->nx == 1) { tx = 1; ty = 16; }
+        if (grid->ny == 1) { tx = 8; ty = 1; }
+        else if (grid->nx == 1) { tx = 1; ty = 8; }
         else { tx = 4; ty = 2; }
     }
     PSKTM_CHECK(tx * ty == 8 || tx * ty == 16, PSKTM_E_INVAL,
