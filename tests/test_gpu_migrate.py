@@ -357,3 +357,61 @@ def test_gained_precise_unsupported_and_batch(cuda):
     plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, precise=True)
     with pytest.raises(psktm.PsktmError):
         psktm.migrate(plan, t(tr), t(vel), gained=True)
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_lines_take_1d_tiles_by_default(cuda, axis):
+    """A line along x (ny = 1) or y (nx = 1) gets 8x1 or 1x8 tiles by default:
+    the default plan's image is bit-identical to that explicit tile shape,
+    matches the FP64 oracle, and passes the precise-mode dot-product test."""
+    import torch
+    nx, ny = (40, 1) if axis == "x" else (1, 40)
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=21, ntr=1500, nx=nx, ny=ny)
+    vt = (8, 1) if axis == "x" else (1, 8)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap)
+    img = psktm.migrate(plan, t(tr), t(vel))
+    plan_t, _ = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, tile_x=vt[0], tile_y=vt[1])
+    img_t = psktm.migrate(plan_t, t(tr), t(vel))
+    assert torch.equal(img.view(torch.int32), img_t.view(torch.int32))
+    ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+    assert rel_l2(img.cpu().numpy(), ref) <= 1e-4
+    pp, _ = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, precise=True)
+    g = torch.Generator(device="cpu").manual_seed(2)
+    d = torch.randn(tr.shape, generator=g).to(cuda)
+    m = torch.randn(grid[0], grid[3], grid[6], generator=g).to(cuda)
+    lhs = psktm.dot_f64(psktm.migrate(pp, d, t(vel)).reshape(-1), m.reshape(-1)).item()
+    rhs = psktm.dot_f64(d.reshape(-1), psktm.demigrate(pp, m, t(vel)).reshape(-1)).item()
+    print(f"line along {axis}: dot-product test rel={abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+
+
+def test_longest_rows_and_rows_too_long_for_lds(cuda):
+    """Traces near the LDS limit (one staged row of 16,000 samples per
+    workgroup in the forward, one FP64 accumulator row in the adjoint) match
+    the FP64 oracle with traveltimes reaching the end of the rows; rows longer
+    than the LDS holds are refused with an error, not launched.
+
+    Tolerance 5e-4 rather than 1e-4: an FP32 time of ~15,000 samples resolves
+    only ~1e-3 of a sample (2^-24 x, summed over the two square roots), and
+    the random sinusoids here reach 0.06 cycles per sample, so interpolated
+    values at the end of the rows carry up to ~2 pi 0.06 1e-3 = 4e-4 relative
+    error (measured 1.4e-4 over the image)."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=23, ntr=300, nx=3, ny=2, nt=200, ns=16000)
+    grid = grid[:8] + (0.15,)               # tau to 30 s: times up to ~60 s of 64 s
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap)
+    img = psktm.migrate(plan, t(tr), t(vel)).cpu().numpy()
+    ref = oracle_f64(grid, mid, off, tr, vel, dt, ap, oracle.LINEAR)
+    assert rel_l2(img, ref) <= 5e-4
+    m = torch.as_tensor(ref.astype(np.float32)).to(cuda)
+    back = psktm.demigrate(plan, m, t(vel))
+    assert torch.isfinite(back).all() and back.abs().max() > 0
+    lhs = psktm.dot_f64(psktm.migrate(plan, t(tr), t(vel)).reshape(-1), m.reshape(-1)).item()
+    rhs = psktm.dot_f64(t(tr).reshape(-1), back.reshape(-1)).item()
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    g2, mid2, off2, tr2, vel2, dt2, ap2 = small_3d(seed=24, ntr=20, nx=3, ny=2, nt=50, ns=30000)
+    plan2, t2 = gpu_plan(cuda, g2, mid2, off2, tr2.shape[1], dt2, ap2)
+    with pytest.raises(psktm.PsktmError):
+        psktm.migrate(plan2, t2(tr2), t2(vel2))
+    with pytest.raises(psktm.PsktmError):
+        psktm.demigrate(plan2, torch.zeros(3, 2, 50, device=cuda), t2(vel2))
