@@ -154,7 +154,7 @@ def load() -> ctypes.CDLL:
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run "
-                          "`make -C seismic-migration_amd/csrc` or __graft_entry__.build()")
+                          "`make -C seismic-migration_amd/csrc`")
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(L, name, None)

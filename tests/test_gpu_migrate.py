@@ -2,7 +2,7 @@
 
 Tolerances (relative L2 over the compared image, FP64 reference):
   LINEAR + FAST + FP32 traces   <= 1e-4
-  LINEAR + FAST + FP16 traces   <= 1e-3   (the BASELINE metric's bound)
+  LINEAR + FAST + FP16 traces   <= 1e-3   (the headline metric's bound)
   LINEAR + FAST + BF16 traces   <= 8e-3   (reported form, 8-bit mantissa)
   TRUNC  + FAST + FP32 traces   <= 2e-2   (index flips near sample
                                            boundaries; smooth data)
