@@ -196,8 +196,9 @@ typedef struct psktm_options {
     int   tile_x, tile_y;     /* bins per tile (8 or 16 in all); 0 = library
                                  choice: 4x2 on a 3-D grid, 8x1 on a line */
     int   traces_per_stage;   /* traces staged in LDS at once (1-8); 0 = the
-                                 largest up to 5 that keeps the kernel's
-                                 workgroups per CU */
+                                 largest up to 5 (8 for gained FP16 traces,
+                                 whose rows take half the LDS) that keeps the
+                                 kernel's workgroups per CU */
     int   precise;            /* 1: compensated (Kahan) FP32 accumulation in
                                  the forward, for least squares and adjoint
                                  tests. 0: plain FP32 accumulation. The

@@ -21,7 +21,7 @@
 namespace psktm {
 namespace {
 
-template <int NTH, int TB, int TL, bool KH, bool GN>
+template <int NTH, int TB, int TL, bool KH, bool GN, bool HP>
 __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__restrict__ descs,
                                                             const int2 *__restrict__ items,
                                                             int n_items,
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__res
 {
     extern __shared__ __align__(16) float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
-    __shared__ unsigned s_mask[kGMax];
+    __shared__ unsigned s_mask[2 * kGMax];   // aperture bits, then trace scales (HP)
     __shared__ int s_item;
     for (;;) {
         __syncthreads();                 // LDS of the previous item is free
@@ -40,8 +40,8 @@ __global__ __launch_bounds__(NTH) void migrate_batch_kernel(const FwdArgs *__res
         const int g = __builtin_amdgcn_readfirstlane(items[it].x);
         const int tc = __builtin_amdgcn_readfirstlane(items[it].y);
         const FwdArgs a = descs[g];
-        fwd_tile<NTH, TB, TL, false, KH, GN>(a, tc / a.n_chunks, tc % a.n_chunks, 0, s_pairs,
-                                         s_ab, s_mask);
+        fwd_tile<NTH, TB, TL, false, KH, GN, HP>(a, tc / a.n_chunks, tc % a.n_chunks, 0, s_pairs,
+                                             s_ab, s_mask);
     }
 }
 
@@ -61,39 +61,41 @@ __global__ void batch_tables_kernel(int nt, float tau0, float dtau, float inv_dt
 
 using BatchFn = void (*)(const FwdArgs *, const int2 *, int, unsigned *);
 
-template <int NTH, int TB, int TL, bool KH, bool GN>
+template <int NTH, int TB, int TL, bool KH, bool GN, bool HP>
 BatchFn batch_pick()
 {
-    if constexpr (acc_regs(TB, TL, false, KH) <= reg_budget(KH) && !(KH && GN))
-        return migrate_batch_kernel<NTH, TB, TL, KH, GN>;
+    if constexpr (acc_regs(TB, TL, false, KH) <= reg_budget(KH) && !(KH && GN) && (GN || !HP))
+        return migrate_batch_kernel<NTH, TB, TL, KH, GN, HP>;
     else
         return nullptr;
 }
 
-template <int NTH, int TB, bool KH, bool GN>
+template <int NTH, int TB, bool KH, bool GN, bool HP = false>
 BatchFn batch_by_tl(int tl)
 {
     switch (tl) {
-    case 2: return batch_pick<NTH, TB, 2, KH, GN>();
-    case 3: return batch_pick<NTH, TB, 3, KH, GN>();
-    case 4: return batch_pick<NTH, TB, 4, KH, GN>();
-    case 6: return batch_pick<NTH, TB, 6, KH, GN>();
-    case 8: return batch_pick<NTH, TB, 8, KH, GN>();
+    case 2: return batch_pick<NTH, TB, 2, KH, GN, HP>();
+    case 3: return batch_pick<NTH, TB, 3, KH, GN, HP>();
+    case 4: return batch_pick<NTH, TB, 4, KH, GN, HP>();
+    case 6: return batch_pick<NTH, TB, 6, KH, GN, HP>();
+    case 8: return batch_pick<NTH, TB, 8, KH, GN, HP>();
     default: return nullptr;
     }
 }
 
+// hp: gained FP16 traces staged as half pairs
 template <int NTH, int TB>
-BatchFn batch_by_mode(int tl, bool kh, bool gn)
+BatchFn batch_by_mode(int tl, bool kh, bool gn, bool hp)
 {
     if (kh) return batch_by_tl<NTH, TB, true, false>(tl);
+    if (gn && hp) return batch_by_tl<NTH, TB, false, true, true>(tl);
     return gn ? batch_by_tl<NTH, TB, false, true>(tl) : batch_by_tl<NTH, TB, false, false>(tl);
 }
 
-BatchFn batch_kernel(int nth, int tb, int tl, bool kh, bool gn)
+BatchFn batch_kernel(int nth, int tb, int tl, bool kh, bool gn, bool hp = false)
 {
-    if (nth == 256) return tb == 8 ? batch_by_mode<256, 8>(tl, kh, gn) : batch_by_mode<256, 16>(tl, kh, gn);
-    return tb == 8 ? batch_by_mode<512, 8>(tl, kh, gn) : batch_by_mode<512, 16>(tl, kh, gn);
+    if (nth == 256) return tb == 8 ? batch_by_mode<256, 8>(tl, kh, gn, hp) : batch_by_mode<256, 16>(tl, kh, gn, hp);
+    return tb == 8 ? batch_by_mode<512, 8>(tl, kh, gn, hp) : batch_by_mode<512, 16>(tl, kh, gn, hp);
 }
 
 } // namespace
@@ -109,9 +111,11 @@ struct psktm_batch {
     psktm::FwdShape shape{};
     psktm::BatchFn fn = nullptr;        // raw traces
     psktm::BatchFn fn_gained = nullptr; // gained traces (same shape), NULL in precise mode
+    psktm::BatchFn fn_half = nullptr;   // gained FP16 traces as half pairs (same shape)
     int G = 4, nsp = 0, lim = 0;
-    size_t lds = 0;
-    int blocks = 0;
+    int G_h = 4, nsp_h = 0, lim_h = 0;  // the half-pair kernel's stage, rows and limit
+    size_t lds = 0, lds_h = 0;
+    int blocks = 0, blocks_h = 0;
     int2 *d_items = nullptr;
     int n_items = 0;
     unsigned *d_counter = nullptr;
@@ -142,7 +146,7 @@ void batch_free(psktm_batch *b)
     delete b;
 }
 
-psktm_status enqueue(psktm_batch *b, const float *v, bool gained, hipStream_t st)
+psktm_status enqueue(psktm_batch *b, const float *v, bool gained, bool hp, hipStream_t st)
 {
     const double inv_dt = 1.0 / (double)b->dt;
     hipLaunchKernelGGL(batch_tables_kernel, dim3((b->nt + 255) / 256), dim3(256), 0, st, b->nt,
@@ -150,8 +154,9 @@ psktm_status enqueue(psktm_batch *b, const float *v, bool gained, hipStream_t st
                        b->tsq);
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, b->d_counter);
     if (b->n_items > 0)
-        hipLaunchKernelGGL(gained ? b->fn_gained : b->fn, dim3(b->blocks), dim3(b->shape.nth),
-                           b->lds, st,
+        hipLaunchKernelGGL(hp ? b->fn_half : (gained ? b->fn_gained : b->fn),
+                           dim3(hp ? b->blocks_h : b->blocks), dim3(b->shape.nth),
+                           hp ? b->lds_h : b->lds, st,
                            (const FwdArgs *)b->d_args, (const int2 *)b->d_items, b->n_items,
                            b->d_counter);
     return check_launch("migrate_batch_kernel");
@@ -216,7 +221,7 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
     b->tb = b->plans[0]->tile_x * b->plans[0]->tile_y;
     const bool kh = opt->precise != 0;
     psktm::FwdLaunch L{};
-    st = choose_launch(b->nt, b->ns, b->tb, false, kh, opt->traces_per_stage, opt->row_pad,
+    st = choose_launch(b->nt, b->ns, b->tb, false, kh, false, opt->traces_per_stage, opt->row_pad,
                        [&](int nth, int tl) { return (const void *)batch_kernel(nth, b->tb, tl, kh, false); },
                        &L);
     if (st != PSKTM_OK) {
@@ -235,6 +240,31 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
         (b->fn_gained && (st = ensure_dynamic_lds((const void *)b->fn_gained, b->lds)) != PSKTM_OK)) {
         batch_free(b);
         return st;
+    }
+    if (!kh) {
+        // the half-pair kernel of the same shape, with its own stage size
+        psktm::FwdLaunch H{};
+        const psktm::FwdShape sh = L.shape;
+        st = choose_launch(b->nt, b->ns, b->tb, false, false, true, opt->traces_per_stage,
+                           opt->row_pad,
+                           [&](int nth, int tl) {
+                               return (nth == sh.nth && tl == sh.tl)
+                                   ? (const void *)batch_kernel(nth, b->tb, tl, false, true, true)
+                                   : nullptr;
+                           },
+                           &H);
+        if (st == PSKTM_OK && H.shape.nth == sh.nth && H.shape.tl == sh.tl) {
+            b->fn_half = (BatchFn)H.fn;
+            b->G_h = H.G;
+            b->nsp_h = H.nsp;
+            b->lim_h = opt->row_pad < 0 ? 0 : pair_limit(H.nsp);
+            b->lds_h = H.lds;
+            st = ensure_dynamic_lds((const void *)b->fn_half, b->lds_h);
+        }
+        if (st != PSKTM_OK) {
+            batch_free(b);
+            return st;
+        }
     }
     // work items, longest first: cost = pairs of the tile x its tau pass
     std::vector<std::pair<double, int2>> items;
@@ -280,6 +310,12 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
                                                           b->shape.nth, b->lds)) != hipSuccess)
         return fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     b->blocks = std::max(1, std::min(b->n_items, std::max(1, per_cu) * cus));
+    if (b->fn_half) {
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b->fn_half,
+                                                              b->shape.nth, b->lds_h)) != hipSuccess)
+            return fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        b->blocks_h = std::max(1, std::min(b->n_items, std::max(1, per_cu) * cus));
+    }
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e, "hipStreamSynchronize");
     *batch = b;
     return PSKTM_OK;
@@ -317,6 +353,9 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
     PSKTM_CHECK(tr->gained == 0 || tr->gained == 1, PSKTM_E_INVAL, "gained must be 0 or 1");
     PSKTM_CHECK(!tr->gained || b->fn_gained, PSKTM_E_UNSUPPORTED,
                 "precise mode takes raw traces (gained = 0)");
+    const bool hp = tr->gained && tr->dtype == PSKTM_F16 && b->fn_half;
+    PSKTM_CHECK(!hp || ((uintptr_t)tr->data & 1u) == 0, PSKTM_E_INVAL,
+                "FP16 trace data must be 2-byte aligned");
     hipStream_t st = as_stream(stream);
     const void *key[5] = {tr->data, tr->scale, vel->v, images,
                           (const void *)(intptr_t)(tr->dtype + 1 + 16 * tr->gained)};
@@ -335,9 +374,9 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
         a.dtype = tr->dtype;
         a.gained = tr->gained;
         if (tr->gained) a.wscale = 1.0f;
-        a.G = b->G;
-        a.nsp = b->nsp;
-        a.lim = b->lim;
+        a.G = hp ? b->G_h : b->G;
+        a.nsp = hp ? b->nsp_h : b->nsp;
+        a.lim = hp ? b->lim_h : b->lim;
         a.accumulate = 0;
         a.n_split = 1;
         a.traces = (const char *)tr->data + (size_t)b->gathers[g].first_trace * b->ns * esz;
@@ -352,7 +391,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
                                  hipMemcpyHostToDevice, st));
         // h_args must outlive the copy: wait for it before returning
         PSKTM_HIP(hipStreamSynchronize(st));
-        return enqueue(b, vel->v, tr->gained != 0, st);
+        return enqueue(b, vel->v, tr->gained != 0, hp, st);
     }
     // (re)capture for these pointers. Recording happens on a private
     // non-blocking stream (the caller's stream may be the legacy default
@@ -365,7 +404,7 @@ psktm_status psktm_batch_migrate(psktm_batch *b, const psktm_traces *tr, const p
     PSKTM_HIP(hipMemcpy(b->d_args, b->h_args.data(), sizeof(FwdArgs) * b->h_args.size(),
                         hipMemcpyHostToDevice));
     PSKTM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    psktm_status s = enqueue(b, vel->v, tr->gained != 0, b->cap);
+    psktm_status s = enqueue(b, vel->v, tr->gained != 0, hp, b->cap);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap, &graph);
     if (s != PSKTM_OK) {

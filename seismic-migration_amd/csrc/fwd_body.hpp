@@ -50,6 +50,22 @@ __host__ __device__ inline int pair_stride(int ns, int pad = 0)
     return (ns + pad + 2 * kPairFront + 1) & ~1;
 }
 
+// Row stride of the half pair table (FP16 traces, gained form): entry n is
+// one dword, (h[n], h[n+1] - h[n]) as two FP16 values (low, high half), with
+// the per-trace scale applied in the accumulate. Rows are a multiple of 4
+// entries so that every row starts 16-byte aligned.
+__host__ __device__ inline int half_stride(int ns, int pad = 0)
+{
+    return (ns + pad + 2 * kPairFront + 3) & ~3;
+}
+
+__host__ __device__ inline int row_stride(int ns, int pad, bool hp)
+{
+    return hp ? half_stride(ns, pad) : pair_stride(ns, pad);
+}
+
+__host__ __device__ inline int entry_bytes(bool hp) { return hp ? 4 : 8; }
+
 // Entries from n = 0 to the end of a row of stride nsp.
 __host__ __device__ inline int pair_limit(int nsp) { return nsp - kPairFront; }
 
@@ -140,6 +156,62 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
     }
 }
 
+// Stages rows s .. s+G-1 of FP16 traces as half pairs: the same items,
+// chunks and buffer ranges as fill_stage, each lane writing the 8 entries of
+// its chunk as dwords (h[n] | (h[n+1] - h[n]) << 16, one FP16 rounding of the
+// difference), zero outside the valid range. Rows start 16-byte aligned, so a
+// chunk whose first entry is a multiple of 4 writes two 16-byte stores.
+template <int NTH>
+__device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, int G,
+                                               unsigned *__restrict__ s_half)
+{
+    constexpr int W = NTH / 64;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int groups = ((14 + a.ns * 2) / 16 + 1 + 63) / 64;
+    const bool lin = a.interp == PSKTM_INTERP_LINEAR;
+    const int lo = lin ? 0 : 1, hi = lin ? a.ns - 2 : a.ns - 1;
+    for (int it = wave; it < G * groups; it += W) {
+        const int g = it / groups, cg = it - g * groups;
+        const long long q = s + g;
+        const int k = a.all_traces ? (int)q : a.list[q];
+        const uintptr_t ra = (uintptr_t)a.traces + (size_t)k * a.ns * 2;
+        const int sb = (int)(ra & 15u);
+        const int n_chunks = (sb + a.ns * 2) / 16 + 1;
+        const int c = cg * 64 + lane;
+        if (c >= n_chunks) continue;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(ra & ~(uintptr_t)15), (short)0, (sb + a.ns * 2 + 3) & ~3, 0x00020000);
+        const int n0 = (16 * c - sb) >> 1;          // rows start on a whole sample
+        unsigned *P = s_half + g * a.nsp + kPairFront + n0;
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * c, 0, 0);
+        const unsigned nx = __builtin_amdgcn_raw_buffer_load_b16(rs, 16 * c + 16, 0, 0);
+        const unsigned lm = lin ? 0xffffu : 0u;     // TRUNC: no difference term
+        auto entry = [&](int i) -> unsigned {
+            const unsigned w0 = (v[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+            const unsigned w1 = i < 7 ? ((v[(i + 1) >> 1] >> (16 * ((i + 1) & 1))) & 0xffffu) : nx;
+            const unsigned dd = (unsigned)__half_as_ushort(
+                __hsub(__ushort_as_half((unsigned short)w1), __ushort_as_half((unsigned short)w0)));
+            return w0 | ((dd & lm) << 16);
+        };
+        // whole waves of interior chunks on 16-byte aligned entries (the
+        // alignment is the same for every chunk of a row): no range tests
+        const bool inner = (sb & 6) == 0 && n0 >= lo && n0 + 7 <= hi;
+        if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; i += 4)
+                reinterpret_cast<uint4 *>(P)[i >> 2] =
+                    make_uint4(entry(i), entry(i + 1), entry(i + 2), entry(i + 3));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const bool ok = (unsigned)(n0 + i - lo) <= (unsigned)(hi - lo);
+                P[i] = ok ? entry(i) : 0u;
+            }
+        }
+    }
+}
+
 // Distance terms and aperture bit of (trace geometry ge, bin (i, j)).
 __device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, float dx,
                                            float y0, float dy, int use_ap, float r2,
@@ -158,7 +230,7 @@ __device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, fl
 // sqrt(n dt)), accumulated without the per-update sqrt(t) weight; a template
 // parameter so that each form keeps its own register allocation. Shared by the per-plan kernel and the
 // persistent batch kernel; the caller provides the LDS arrays.
-template <int NTH, int TB, int TL, bool V3, bool KH, bool GN>
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP = false>
 __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const int chunk,
                                          const int split,
                                          float2 *__restrict__ s_pairs,
@@ -198,7 +270,12 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
 
     // The pair table is cleared once per tile: entries past what the fill
     // writes (-7 .. ns+7 of each row) then read as zeros for the whole tile.
-    for (int q = tid; q < a.G * a.nsp; q += NTH) s_pairs[q] = make_float2(0.f, 0.f);
+    if (HP) {
+        unsigned *h = reinterpret_cast<unsigned *>(s_pairs);
+        for (int q = tid; q < a.G * a.nsp; q += NTH) h[q] = 0u;
+    } else {
+        for (int q = tid; q < a.G * a.nsp; q += NTH) s_pairs[q] = make_float2(0.f, 0.f);
+    }
     // Traveltime bound of the tile: every (trace, in-aperture bin) pair has
     // A, B <= reach^2, so x <= 2 sqrt(tau'^2 + reach^2 s') at each tau. When
     // that stays inside the padded rows (with a margin far above the FP32
@@ -251,15 +328,26 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             if ((tid % TB) == 0) {
                 const int shift = (tid & 63) - ((tid & 63) % TB);
                 s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
+                // half pairs hold the stored values: the trace's scale
+                if (HP) {
+                    const int k = q < s1 ? (a.all_traces ? (int)q : a.list[q]) : 0;
+                    s_mask[kGMax + g] = __float_as_uint(q < s1 ? a.scale[k] : 0.f);
+                }
             }
         }
-        fill_stage<NTH>(a, s, s1, s_pairs);
+        if (HP)
+            fill_half_rows<NTH>(a, s, (int)((s1 - s) < (long long)a.G ? (s1 - s) : (long long)a.G),
+                                reinterpret_cast<unsigned *>(s_pairs));
+        else
+            fill_stage<NTH>(a, s, s1, s_pairs);
         __syncthreads();
         // ---- compute --------------------------------------------------------
         for (int g = 0; g < G; ++g) {
             const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
             if (!m || s + g >= s1) continue;
             const float2 *P = s_pairs + g * a.nsp + kPairFront;
+            const unsigned *PH = reinterpret_cast<const unsigned *>(s_pairs) + g * a.nsp + kPairFront;
+            const int sck = HP ? __builtin_amdgcn_readfirstlane((int)s_mask[kGMax + g]) : 0;
 #if defined(__HIP_DEVICE_COMPILE__)
             // LDS address of this trace's distance terms, moved into a VGPR
             // once per trace by an opaque move: the compiler then reads every
@@ -289,9 +377,18 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                     if (CL) x = fminf(x, a.nsf);
                     const int n = (int)x;
                     const float f = __builtin_amdgcn_fractf(x);
-                    const float2 pr = P[n];
-                    const float d = fmaf(f, pr.y, pr.x);
-                    if (KH) {
+                    float d;
+                    if (HP) {
+                        const unsigned e = PH[n];
+                        d = fmaf(f, __half2float(__ushort_as_half((unsigned short)(e >> 16))),
+                                 __half2float(__ushort_as_half((unsigned short)(e & 0xffffu))));
+                    } else {
+                        const float2 pr = P[n];
+                        d = fmaf(f, pr.y, pr.x);
+                    }
+                    if (HP) {
+                        acc[b][j] = fmaf(d, __int_as_float(sck), acc[b][j]);
+                    } else if (KH) {
                         float &c = cmp[KH ? b : 0][KH ? j : 0];
                         const float y = fmaf(d, __builtin_amdgcn_sqrtf(x), -c);
                         const float t = acc[b][j] + y;
@@ -378,16 +475,18 @@ inline FwdShape choose_shape(int nt, int tb, bool v3, bool kh)
 // Zero entries appended to every staged row (psktm_options.row_pad): -1 none,
 // > 0 that many, 0 the largest of ns/4, ns/8, ... (at least 16) whose LDS
 // footprint keeps the kernel's resident workgroups per CU.
-inline psktm_status choose_row_pad(int opt_pad, int ns, int G, const void *fn, int nth, int *pad)
+inline psktm_status choose_row_pad(int opt_pad, int ns, int G, bool hp, const void *fn, int nth,
+                                   int *pad)
 {
     *pad = 0;
     if (opt_pad < 0) return PSKTM_OK;
     if (opt_pad > 0) { *pad = opt_pad; return PSKTM_OK; }
-    const size_t base_lds = (size_t)G * pair_stride(ns) * sizeof(float2);
+    const size_t eb = entry_bytes(hp);
+    const size_t base_lds = (size_t)G * row_stride(ns, 0, hp) * eb;
     int base = 0;
     PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&base, fn, nth, base_lds));
     for (int cand = ns / 4; cand >= 16; cand /= 2) {
-        const size_t lds = (size_t)G * pair_stride(ns, cand) * sizeof(float2);
+        const size_t lds = (size_t)G * row_stride(ns, cand, hp) * eb;
         if (lds > 150 * 1024) continue;
         int occ = 0;
         PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nth, lds));
@@ -421,10 +520,11 @@ struct FwdLaunch {
 // `lookup(nth, tl)` returns the kernel of that shape or nullptr. `opt_G` > 0
 // and `opt_pad` != 0 override the choices.
 template <typename Lookup>
-inline psktm_status choose_launch(int nt, int ns, int tb, bool v3, bool kh, int opt_G, int opt_pad,
-                                  Lookup lookup, FwdLaunch *out)
+inline psktm_status choose_launch(int nt, int ns, int tb, bool v3, bool kh, bool hp, int opt_G,
+                                  int opt_pad, Lookup lookup, FwdLaunch *out)
 {
-    const size_t slot = (size_t)pair_stride(ns) * sizeof(float2);
+    const size_t eb = entry_bytes(hp);
+    const size_t slot = (size_t)row_stride(ns, 0, hp) * eb;
     FwdShape best{0, 0, 0};
     const void *best_fn = nullptr;
     long long best_waves = -1, best_waste = 0;
@@ -457,11 +557,12 @@ inline psktm_status choose_launch(int nt, int ns, int tb, bool v3, bool kh, int 
     int G = opt_G > 0 ? std::min(opt_G, kGMax) : 0;
     if (G == 0) {
         // rows as long as an explicit pad makes them
-        const size_t gslot = (size_t)pair_stride(ns, std::max(opt_pad, 0)) * sizeof(float2);
+        const size_t gslot = (size_t)row_stride(ns, std::max(opt_pad, 0), hp) * eb;
         int reg_blocks = 0;
         PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&reg_blocks, best_fn, best.nth, slot));
         G = 1;
-        for (int g = 5; g >= 1; --g) {
+        // half pairs take half the LDS per row: up to kGMax rows per stage
+        for (int g = hp ? kGMax : 5; g >= 1; --g) {
             int b = 0;
             if ((size_t)g * gslot > 150 * 1024) continue;
             PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, best_fn, best.nth, g * gslot));
@@ -471,11 +572,11 @@ inline psktm_status choose_launch(int nt, int ns, int tb, bool v3, bool kh, int 
     PSKTM_CHECK((size_t)G * slot <= 150 * 1024, PSKTM_E_UNSUPPORTED,
                 "traces of %d samples do not fit in LDS", ns);
     int pad = 0;
-    if ((s = choose_row_pad(opt_pad, ns, G, best_fn, best.nth, &pad)) != PSKTM_OK) return s;
-    const int nsp = pair_stride(ns, pad);
-    PSKTM_CHECK((size_t)G * nsp * sizeof(float2) <= 150 * 1024, PSKTM_E_UNSUPPORTED,
+    if ((s = choose_row_pad(opt_pad, ns, G, hp, best_fn, best.nth, &pad)) != PSKTM_OK) return s;
+    const int nsp = row_stride(ns, pad, hp);
+    PSKTM_CHECK((size_t)G * nsp * eb <= 150 * 1024, PSKTM_E_UNSUPPORTED,
                 "row_pad %d does not fit in LDS", pad);
-    *out = FwdLaunch{best, best_fn, G, pad, nsp, (size_t)G * nsp * sizeof(float2)};
+    *out = FwdLaunch{best, best_fn, G, pad, nsp, (size_t)G * nsp * eb};
     return PSKTM_OK;
 }
 

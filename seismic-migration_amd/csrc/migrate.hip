@@ -19,6 +19,10 @@
 //             is completed by one multiply per sample at the end. Gained
 //             traces (sample n already times sqrt(n dt)) accumulate d alone
 //             (GN instantiation: one v_sqrt_f32 fewer) and end with tau.
+//             Gained FP16 traces (HP) stage half pairs instead, one dword
+//             (h[n], fp16(h[n+1]-h[n])) per entry of the stored values: half
+//             the LDS per row, twice the traces per stage, interpolation by
+//             v_fma_mix_f32 and the trace's scale applied in the accumulate.
 // Out-of-range samples need no test: the pair table's entries outside the
 // valid range of the interpolation rule are zero (TRUNC: 0 < n < ns,
 // src/psktmCPU.c:82; LINEAR: 0 <= n, n+1 < ns), and x is clamped to ns unless
@@ -63,17 +67,17 @@ struct AdjArgs {
     float *traces;
 };
 
-template <int NTH, int TB, int TL, bool V3, bool KH, bool GN>
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP>
 __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 {
     extern __shared__ __align__(16) float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
-    __shared__ unsigned s_mask[kGMax];
+    __shared__ unsigned s_mask[2 * kGMax];   // aperture bits, then trace scales (HP)
     // block -> (tile in work order, tau pass, split); splits of a tile adjacent
     const int split = blockIdx.x % a.n_split;
     const int rest = blockIdx.x / a.n_split;
-    fwd_tile<NTH, TB, TL, V3, KH, GN>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks, split,
-                                  s_pairs, s_ab, s_mask);
+    fwd_tile<NTH, TB, TL, V3, KH, GN, HP>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks,
+                                          split, s_pairs, s_ab, s_mask);
 }
 
 template <int NTH, int TB, int TL, bool V3, typename Acc>
@@ -257,45 +261,48 @@ __global__ void reduce_partials_kernel(const float *__restrict__ partial, int n_
 using FwdFn = void (*)(FwdArgs);
 using AdjFn = void (*)(AdjArgs);
 
-template <int NTH, int TB, int TL, bool V3, bool KH, bool GN>
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP>
 FwdFn fwd_pick()
 {
-    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH) && !(KH && GN))
-        return migrate_fwd_kernel<NTH, TB, TL, V3, KH, GN>;
+    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH) && !(KH && GN) && (GN || !HP))
+        return migrate_fwd_kernel<NTH, TB, TL, V3, KH, GN, HP>;
     else
         return nullptr;
 }
 
-template <int NTH, int TB, bool V3, bool KH, bool GN>
+template <int NTH, int TB, bool V3, bool KH, bool GN, bool HP = false>
 FwdFn fwd_by_tl(int tl)
 {
     switch (tl) {
-    case 2: return fwd_pick<NTH, TB, 2, V3, KH, GN>();
-    case 3: return fwd_pick<NTH, TB, 3, V3, KH, GN>();
-    case 4: return fwd_pick<NTH, TB, 4, V3, KH, GN>();
-    case 6: return fwd_pick<NTH, TB, 6, V3, KH, GN>();
-    case 8: return fwd_pick<NTH, TB, 8, V3, KH, GN>();
+    case 2: return fwd_pick<NTH, TB, 2, V3, KH, GN, HP>();
+    case 3: return fwd_pick<NTH, TB, 3, V3, KH, GN, HP>();
+    case 4: return fwd_pick<NTH, TB, 4, V3, KH, GN, HP>();
+    case 6: return fwd_pick<NTH, TB, 6, V3, KH, GN, HP>();
+    case 8: return fwd_pick<NTH, TB, 8, V3, KH, GN, HP>();
     default: return nullptr;
     }
 }
 
-// mode: 0 raw traces, 1 precise (compensated), 2 gained traces
+// mode: 0 raw traces, 1 precise (compensated), 2 gained traces, 3 gained
+// FP16 traces staged as half pairs
 template <int NTH, int TB>
 FwdFn fwd_by_flags(int tl, bool v3, int mode)
 {
     if (v3) {
         if (mode == 1) return fwd_by_tl<NTH, TB, true, true, false>(tl);
         if (mode == 2) return fwd_by_tl<NTH, TB, true, false, true>(tl);
+        if (mode == 3) return fwd_by_tl<NTH, TB, true, false, true, true>(tl);
         return fwd_by_tl<NTH, TB, true, false, false>(tl);
     }
     if (mode == 1) return fwd_by_tl<NTH, TB, false, true, false>(tl);
     if (mode == 2) return fwd_by_tl<NTH, TB, false, false, true>(tl);
+    if (mode == 3) return fwd_by_tl<NTH, TB, false, false, true, true>(tl);
     return fwd_by_tl<NTH, TB, false, false, false>(tl);
 }
 
-FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh, bool gn)
+FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh, bool gn, bool hp)
 {
-    const int mode = kh ? 1 : (gn ? 2 : 0);
+    const int mode = kh ? 1 : (gn ? (hp ? 3 : 2) : 0);
     if (nth == 256) return tb == 8 ? fwd_by_flags<256, 8>(tl, v3, mode) : fwd_by_flags<256, 16>(tl, v3, mode);
     return tb == 8 ? fwd_by_flags<512, 8>(tl, v3, mode) : fwd_by_flags<512, 16>(tl, v3, mode);
 }
@@ -352,10 +359,14 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     const bool v3 = vel.kind == PSKTM_VEL_3D;
     const bool kh = p->opt.precise != 0;
     const bool gn = tr.gained != 0;
+    // gained FP16 traces: half pairs in LDS (half the bytes per staged row)
+    const bool hp = gn && tr.dtype == PSKTM_F16;
+    PSKTM_CHECK(!hp || ((uintptr_t)tr.data & 1u) == 0, PSKTM_E_INVAL,
+                "FP16 trace data must be 2-byte aligned");
     FwdLaunch L{};
-    if ((s = choose_launch(g.nt, p->n_samples, tb, v3, kh, p->opt.traces_per_stage,
+    if ((s = choose_launch(g.nt, p->n_samples, tb, v3, kh, hp, p->opt.traces_per_stage,
                            p->opt.row_pad,
-                           [&](int nth, int tl) { return (const void *)fwd_kernel(nth, tb, tl, v3, kh, gn); },
+                           [&](int nth, int tl) { return (const void *)fwd_kernel(nth, tb, tl, v3, kh, gn, hp); },
                            &L)) != PSKTM_OK)
         return s;
     const FwdShape sh = L.shape;
