@@ -114,6 +114,7 @@ struct psktm_batch {
     psktm::BatchFn fn_half = nullptr;   // gained FP16 traces as half pairs (same shape)
     int G = 4, nsp = 0, lim = 0;
     int G_h = 4, nsp_h = 0, lim_h = 0;  // the half-pair kernel's stage, rows and limit
+    int nth_h = 0;                      // and its workgroup size
     size_t lds = 0, lds_h = 0;
     int blocks = 0, blocks_h = 0;
     int2 *d_items = nullptr;
@@ -155,7 +156,7 @@ psktm_status enqueue(psktm_batch *b, const float *v, bool gained, bool hp, hipSt
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, b->d_counter);
     if (b->n_items > 0)
         hipLaunchKernelGGL(hp ? b->fn_half : (gained ? b->fn_gained : b->fn),
-                           dim3(hp ? b->blocks_h : b->blocks), dim3(b->shape.nth),
+                           dim3(hp ? b->blocks_h : b->blocks), dim3(hp ? b->nth_h : b->shape.nth),
                            hp ? b->lds_h : b->lds, st,
                            (const FwdArgs *)b->d_args, (const int2 *)b->d_items, b->n_items,
                            b->d_counter);
@@ -242,18 +243,28 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
         return st;
     }
     if (!kh) {
-        // the half-pair kernel of the same shape, with its own stage size
+        // the half-pair kernel: its own shape when that covers tau in as many
+        // passes (the work items are per pass), else the same shape; its own
+        // stage size either way
         psktm::FwdLaunch H{};
         const psktm::FwdShape sh = L.shape;
         st = choose_launch(b->nt, b->ns, b->tb, false, false, true, opt->traces_per_stage,
                            opt->row_pad,
                            [&](int nth, int tl) {
-                               return (nth == sh.nth && tl == sh.tl)
-                                   ? (const void *)batch_kernel(nth, b->tb, tl, false, true, true)
-                                   : nullptr;
+                               return (const void *)batch_kernel(nth, b->tb, tl, false, true, true);
                            },
                            &H);
-        if (st == PSKTM_OK && H.shape.nth == sh.nth && H.shape.tl == sh.tl) {
+        if (st == PSKTM_OK && H.shape.chunks != sh.chunks)
+            st = choose_launch(b->nt, b->ns, b->tb, false, false, true, opt->traces_per_stage,
+                               opt->row_pad,
+                               [&](int nth, int tl) {
+                                   return (nth == sh.nth && tl == sh.tl)
+                                       ? (const void *)batch_kernel(nth, b->tb, tl, false, true, true)
+                                       : nullptr;
+                               },
+                               &H);
+        if (st == PSKTM_OK && H.shape.chunks == sh.chunks) {
+            b->nth_h = H.shape.nth;
             b->fn_half = (BatchFn)H.fn;
             b->G_h = H.G;
             b->nsp_h = H.nsp;
@@ -312,7 +323,7 @@ psktm_status psktm_batch_create(psktm_batch **batch, const psktm_gather *gathers
     b->blocks = std::max(1, std::min(b->n_items, std::max(1, per_cu) * cus));
     if (b->fn_half) {
         if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b->fn_half,
-                                                              b->shape.nth, b->lds_h)) != hipSuccess)
+                                                              b->nth_h, b->lds_h)) != hipSuccess)
             return fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
         b->blocks_h = std::max(1, std::min(b->n_items, std::max(1, per_cu) * cus));
     }

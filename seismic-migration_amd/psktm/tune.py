@@ -22,6 +22,7 @@ Candidate = Tuple[int, int, int]
 # 3-D patches: square-ish tiles first; 2-D lines use (16,1) / (8,1).
 CANDIDATES_3D: Sequence[Candidate] = (
     (4, 4, 4), (4, 4, 2), (4, 4, 8), (8, 2, 4), (2, 8, 4), (4, 2, 4), (2, 4, 4),
+    (8, 2, 8), (2, 8, 8),
 )
 CANDIDATES_2D: Sequence[Candidate] = ((16, 1, 4), (16, 1, 2), (16, 1, 8), (8, 1, 4))
 
