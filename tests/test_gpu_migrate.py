@@ -415,3 +415,45 @@ def test_longest_rows_and_rows_too_long_for_lds(cuda):
         psktm.migrate(plan2, t2(tr2), t2(vel2))
     with pytest.raises(psktm.PsktmError):
         psktm.demigrate(plan2, torch.zeros(3, 2, 50, device=cuda), t2(vel2))
+
+
+@pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
+def test_half_pairs_vs_fp64_gained(cuda, interp):
+    """Gained FP16 traces (staged as half pairs: the stored value and its FP16
+    difference, the trace's scale applied in the accumulate) against the FP64
+    gained operator on exactly the dequantised values the kernel reads. The
+    only rounding beyond the FP32 form is the FP16 difference (2^-11 relative
+    of one interpolation term), so LINEAR keeps the 1e-4 bar of the FP32
+    form; TRUNC reads no difference and keeps its index-flip bar."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=47, ntr=2500, nx=9, ny=7)
+    ns = tr.shape[1]
+    plan, t = gpu_plan(cuda, grid, mid, off, ns, dt, ap, interp=interp)
+    q, sc = psktm.trace_gain(t(tr), dt, out_dtype=torch.float16)
+    img = psktm.migrate(plan, q, t(vel), scale=sc, gained=True).cpu().numpy()
+    deq = (q.float() * sc[:, None]).cpu().numpy()
+    oi = oracle.LINEAR if interp == psktm.INTERP_LINEAR else oracle.TRUNC
+    ref = oracle.pstm_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], deq,
+                          vel, dt, vel3d=True, aperture=ap, interp=oi, gained=True)
+    e = rel_l2(img, ref)
+    print(f"half pairs interp={interp}: rel-L2 vs FP64 gained on the stored values {e:.3e}")
+    assert e <= (1e-4 if interp == psktm.INTERP_LINEAR else 2e-2)
+
+
+def test_half_pairs_stage_sizes_and_padding_bit_identical(cuda):
+    """Half pairs: every lane adds its traces in list order whatever the
+    stage size, and padded rows read zeros past ns like the clamp, so 1, 3
+    and 8 traces per stage, with or without padding, give the same bits
+    (one workgroup per tile: the split-K count depends on the stage size)."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=48, ntr=1800, nx=9, ny=7)
+    ns = tr.shape[1]
+    q, sc = psktm.trace_gain(torch.as_tensor(tr).to(cuda), dt, out_dtype=torch.float16)
+    imgs = []
+    for stage, pad in [(0, 0), (1, 0), (3, 0), (8, 0), (8, -1), (3, 300)]:
+        plan, t = gpu_plan(cuda, grid, mid, off, ns, dt, ap, traces_per_stage=stage, row_pad=pad,
+                           splits=1)
+        imgs.append(psktm.migrate(plan, q, t(vel), scale=sc, gained=True))
+    for im in imgs[1:]:
+        assert torch.equal(im.view(torch.int32), imgs[0].view(torch.int32))
+    assert imgs[0].abs().max() > 0
