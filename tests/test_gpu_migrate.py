@@ -190,6 +190,13 @@ def test_odd_sample_counts_and_unaligned_rows(cuda, ns, offset):
         ref = oracle_f64(grid, mid, off, src, vel, dt, ap, oracle.LINEAR)
         assert np.linalg.norm(ref) > 0
         errs[str(dtype)] = rel_l2(img, ref)
+        if dtype == torch.float16:
+            # the same rows as gained FP16 traces: staged as half pairs
+            img_g = psktm.migrate(plan, view, t(vel), scale=sc, gained=True).cpu().numpy()
+            ref_g = oracle.pstm_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0],
+                                    off[:, 1], src, vel, dt, vel3d=True, aperture=ap,
+                                    interp=oracle.LINEAR, gained=True)
+            errs["float16 half pairs"] = rel_l2(img_g, ref_g)
     print("rel-L2", errs)
     assert max(errs.values()) <= 1e-4, errs
 
