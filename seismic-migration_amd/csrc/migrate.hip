@@ -466,13 +466,23 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     // workgroups (LDS-bound by the FP64 rows); 256 x 8 kept 8 waves per CU
     // and took 66.3 ms on the 2-D line against 48.9 ms, 1024 x 2 74.0 ms
     constexpr int NTH = 512, TL = 4;
-    const AdjFn fn = (tb == 8)
+    // Without an aperture every work item visits every tile, so the adjoint
+    // may take tiles of its own: 8-bin plan tiles are doubled along their
+    // long axis (half the tile visits and barriers; 48.8 vs 49.7 ms on a
+    // 512-CMP line, same output bits there).
+    int atx = p->tile_x, aty = p->tile_y;
+    if (p->all_traces && tb == 8) {
+        if (atx >= aty) atx *= 2; else aty *= 2;
+    }
+    const int atb = atx * aty;
+    const AdjFn fn = (atb == 8)
         ? (v3 ? demigrate_kernel<NTH, 8, TL, true, double> : demigrate_kernel<NTH, 8, TL, false, double>)
         : (v3 ? demigrate_kernel<NTH, 16, TL, true, double> : demigrate_kernel<NTH, 16, TL, false, double>);
     AdjArgs a{};
     a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
     a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;
-    a.tile_x = p->tile_x; a.tile_y = p->tile_y; a.tiles_x = p->tiles_x; a.tiles_y = p->tiles_y;
+    a.tile_x = atx; a.tile_y = aty;
+    a.tiles_x = (g.nx + atx - 1) / atx; a.tiles_y = (g.ny + aty - 1) / aty;
     a.n_chunks = (g.nt + NTH * TL - 1) / (NTH * TL);
     a.cells_y = p->cells_y; a.cell_mx = p->cell_mx; a.cell_my = p->cell_my;
     a.win_x = p->win_x; a.win_y = p->win_y;
