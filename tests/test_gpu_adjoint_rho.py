@@ -91,6 +91,23 @@ def test_adjoint_vs_fp64_oracle(cuda):
     assert rel_l2(got, ref) <= 1e-4
 
 
+@pytest.mark.parametrize("tiles", [(0, 0), (4, 2), (2, 4), (8, 1), (1, 8)])
+def test_adjoint_without_aperture_vs_fp64_oracle(cuda, tiles):
+    """Without an aperture the adjoint doubles 8-bin plan tiles along their
+    long axis (every work item visits every tile): the 3-D adjoint against
+    the FP64 adjoint oracle for the default and each 8-bin orientation."""
+    grid, mid, off, tr, vel, dt, _ = small_3d(seed=14, ntr=300, nx=9, ny=6, nt=160, ns=200)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, 0.0, tile_x=tiles[0],
+                       tile_y=tiles[1])
+    rng = np.random.default_rng(3)
+    m = rng.standard_normal((grid[0], grid[3], grid[6])).astype(np.float32)
+    got = psktm.demigrate(plan, t(m), t(vel)).cpu().numpy()
+    ref = oracle.pstm_adj_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], m,
+                              vel, dt, tr.shape[1], vel3d=True, aperture=0.0,
+                              interp=oracle.LINEAR)
+    assert rel_l2(got, ref) <= 1e-4
+
+
 @pytest.mark.parametrize("seed", [3, 4])
 def test_dot_product_line_2d_no_aperture(cuda, seed):
     """Every trace reaches every image column; error scaled as in the
