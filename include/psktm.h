@@ -10,6 +10,7 @@
  *       fileWrite, errorMessage, loadInputData, load*Config
  *                                            src/ioFunctions.c:4-175
  *       ktmMigrationGPU                      src/psktmCUDA.cu:211-221
+ *       ktmMigrationCPU                      src/psktmCPU.c:13-23
  *     The reference never defines its structs or IBM masks; the layouts
  *     below are this library's definition (fields inferred from their uses).
  *
@@ -115,14 +116,22 @@ void loadRTM2dConfig(void);
 void loadKTMConfig(void);
 void loadKDMConfig(void);
 
-/* Drop-in for src/psktmCUDA.cu:211-221 (and for ktmMigrationCPU,
- * src/psktmCPU.c:13-23, which has the identical list): host arrays in, image
- * zeroed then overwritten, synchronous. Runs the reference-exact HIP form
- * (truncating index, IEEE sqrt/div, no contraction, traces summed in k order
- * per image sample), which is bit-identical to the single-thread reference
- * loop. No aperture (the reference has none). On any failure the image stays
- * zeroed and psktm_last_error() says why. */
+/* Drop-in for src/psktmCUDA.cu:211-221: host arrays in, image zeroed then
+ * overwritten, synchronous. Runs the reference-exact HIP form (truncating
+ * index, IEEE sqrt/div, no contraction, traces summed in k order per image
+ * sample), which is bit-identical to the single-thread reference loop. No
+ * aperture (the reference has none). On any failure the image stays zeroed
+ * and psktm_last_error() says why. */
 void ktmMigrationGPU(struct imageGrid *imageX, struct imageGrid *imageY,
+                     struct imageGrid *imageZ, struct jobParams *config,
+                     float *midX, float *midY, float *offX, float *offY,
+                     float *traces, float *slowness, float *image);
+/* Drop-in for src/psktmCPU.c:13-23 (the identical argument list): the same
+ * semantics and the same image bits as ktmMigrationGPU, which equal the
+ * reference CPU loop run on one thread (its OpenMP loop races on the image,
+ * src/psktmCPU.c:56,84). Computed on the GPU by the same kernel; fails like
+ * ktmMigrationGPU (zeroed image, psktm_last_error()) without a gfx950. */
+void ktmMigrationCPU(struct imageGrid *imageX, struct imageGrid *imageY,
                      struct imageGrid *imageZ, struct jobParams *config,
                      float *midX, float *midY, float *offX, float *offY,
                      float *traces, float *slowness, float *image);

@@ -1,5 +1,6 @@
-// C ABI core of libpsktm.so: error state, device checks, the drop-in
-// ktmMigrationGPU (src/psktmCUDA.cu:211-332) and psktm_migrate_exact.
+// C ABI core of libpsktm.so: error state, device checks, the drop-ins
+// ktmMigrationGPU (src/psktmCUDA.cu:211-332) and ktmMigrationCPU
+// (src/psktmCPU.c:13-101), and psktm_migrate_exact.
 #include "internal.hpp"
 
 #include <cstdarg>
@@ -150,28 +151,35 @@ psktm_status psktm_migrate_exact(const psktm_grid *grid, const psktm_survey *sur
     return launch_exact(*grid, *survey, traces, vel, image, as_stream(stream));
 }
 
-// Drop-in for ktmMigrationGPU (src/psktmCUDA.cu:211-332): host arrays,
-// synchronous, image zeroed first (:241-242) and left zeroed on any failure.
-// Unlike the reference it keeps the device context alive (:327), never prints
-// (:299,312) and has no block limit (:298-302).
-void ktmMigrationGPU(struct imageGrid *imageX, struct imageGrid *imageY,
-                     struct imageGrid *imageZ, struct jobParams *config,
-                     float *midX, float *midY, float *offX, float *offY,
-                     float *traces, float *slowness, float *image)
+} // extern "C"
+
+namespace {
+
+// The legacy 11-argument migration on host arrays, shared by ktmMigrationGPU
+// (src/psktmCUDA.cu:211-332) and ktmMigrationCPU (src/psktmCPU.c:13-101):
+// synchronous, image zeroed first (src/psktmCPU.c:52-53, src/psktmCUDA.cu:
+// 241-242) and left zeroed on any failure, then the reference-exact HIP
+// kernel, which is bit-identical to the single-thread CPU loop. Unlike the
+// CUDA form it keeps the device context alive (:327), never prints (:299,312)
+// and has no block limit (:298-302). `who` names the entry in error texts.
+void legacy_migrate(const char *who, struct imageGrid *imageX, struct imageGrid *imageY,
+                    struct imageGrid *imageZ, struct jobParams *config, float *midX,
+                    float *midY, float *offX, float *offY, float *traces, float *slowness,
+                    float *image)
 {
     psktm__clear_error();
     if (!imageX || !imageY || !imageZ || !config) {
-        psktm__set_error("ktmMigrationGPU: NULL grid or job struct");
+        psktm__set_error("%s: NULL grid or job struct", who);
         return;
     }
     const int nx = imageX->imagePnts, ny = imageY->imagePnts, nt = imageZ->imagePnts;
     if (nx <= 0 || ny <= 0 || nt <= 0) {
-        psktm__set_error("ktmMigrationGPU: image sizes must be positive");
+        psktm__set_error("%s: image sizes must be positive", who);
         return;
     }
     const size_t nimg = (size_t)nx * ny * nt;
     if (!image) {
-        psktm__set_error("ktmMigrationGPU: image is NULL");
+        psktm__set_error("%s: image is NULL", who);
         return;
     }
     memset(image, 0, nimg * sizeof(float));
@@ -181,11 +189,11 @@ void ktmMigrationGPU(struct imageGrid *imageX, struct imageGrid *imageY,
                  imageZ->imageStep};
     const int ntr = config->traceNo, ns = config->tracePts;
     if (ntr < 0 || ns <= 0 || !(config->traceSampleRate > 0.0f) || ns >= (1 << 24)) {
-        psktm__set_error("ktmMigrationGPU: invalid job (traceNo=%d tracePts=%d)", ntr, ns);
+        psktm__set_error("%s: invalid job (traceNo=%d tracePts=%d)", who, ntr, ns);
         return;
     }
     if (!slowness || (ntr > 0 && (!midX || !midY || !offX || !offY || !traces))) {
-        psktm__set_error("ktmMigrationGPU: NULL input array");
+        psktm__set_error("%s: NULL input array", who);
         return;
     }
     if (ntr == 0) return;   // nothing to add: the zeroed image is the answer
@@ -232,6 +240,32 @@ void ktmMigrationGPU(struct imageGrid *imageX, struct imageGrid *imageY,
     if (dTr) (void)hipFree(dTr);
     if (dGeo) (void)hipFree(dGeo);
     if (dVel) (void)hipFree(dVel);
+}
+
+} // namespace
+
+extern "C" {
+
+// src/psktmCUDA.cu:211-221
+void ktmMigrationGPU(struct imageGrid *imageX, struct imageGrid *imageY,
+                     struct imageGrid *imageZ, struct jobParams *config,
+                     float *midX, float *midY, float *offX, float *offY,
+                     float *traces, float *slowness, float *image)
+{
+    legacy_migrate("ktmMigrationGPU", imageX, imageY, imageZ, config, midX, midY, offX, offY,
+                   traces, slowness, image);
+}
+
+// src/psktmCPU.c:13-23: the same call, the same image as the reference's
+// loop run on one thread (its OpenMP loop races on the image,
+// src/psktmCPU.c:56,84), computed by the reference-exact HIP kernel.
+void ktmMigrationCPU(struct imageGrid *imageX, struct imageGrid *imageY,
+                     struct imageGrid *imageZ, struct jobParams *config,
+                     float *midX, float *midY, float *offX, float *offY,
+                     float *traces, float *slowness, float *image)
+{
+    legacy_migrate("ktmMigrationCPU", imageX, imageY, imageZ, config, midX, midY, offX, offY,
+                   traces, slowness, image);
 }
 
 } // extern "C"

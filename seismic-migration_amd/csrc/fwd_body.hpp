@@ -51,9 +51,11 @@ __host__ __device__ inline int pair_stride(int ns, int pad = 0)
 }
 
 // Row stride of the half pair table (FP16 traces, gained form): entry n is
-// one dword, (h[n], h[n+1] - h[n]) as two FP16 values (low, high half), with
-// the per-trace scale applied in the accumulate. Rows are a multiple of 4
-// entries so that every row starts 16-byte aligned.
+// one dword, (h[n]/2, h[n+1]/2 - h[n]/2) as two FP16 values (low, high half),
+// with twice the per-trace scale applied in the accumulate. Halving keeps the
+// difference of any two finite FP16 values finite (|h| <= 65504, so the
+// halved difference is at most 65504). Rows are a multiple of 4 entries so
+// that every row starts 16-byte aligned.
 __host__ __device__ inline int half_stride(int ns, int pad = 0)
 {
     return (ns + pad + 2 * kPairFront + 3) & ~3;
@@ -158,9 +160,12 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
 
 // Stages rows s .. s+G-1 of FP16 traces as half pairs: the same items,
 // chunks and buffer ranges as fill_stage, each lane writing the 8 entries of
-// its chunk as dwords (h[n] | (h[n+1] - h[n]) << 16, one FP16 rounding of the
-// difference), zero outside the valid range. Rows start 16-byte aligned, so a
-// chunk whose first entry is a multiple of 4 writes two 16-byte stores.
+// its chunk as dwords (u[n] | (u[n+1] - u[n]) << 16 with u = h/2, exact for
+// every normal FP16 value, and one FP16 rounding of the difference), zero
+// outside the valid range. Without the halving, neighbours such as +32768 and
+// -32752 (both valid quantised samples) would difference to -inf. Rows start
+// 16-byte aligned, so a chunk whose first entry is a multiple of 4 writes two
+// 16-byte stores.
 template <int NTH>
 __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, int G,
                                                unsigned *__restrict__ s_half)
@@ -186,13 +191,19 @@ __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, in
         unsigned *P = s_half + g * a.nsp + kPairFront + n0;
         const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * c, 0, 0);
         const unsigned nx = __builtin_amdgcn_raw_buffer_load_b16(rs, 16 * c + 16, 0, 0);
+        // entry i from the sample pair (h[n0+i], h[n0+i+1]) as one dword (a
+        // funnel shift for odd i): both halved by one packed FP16 multiply
+        // with an inline constant, then (u0, u1 - u0)
+        typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+        const h2v half2 = {(_Float16)0.5f, (_Float16)0.5f};
         const unsigned lm = lin ? 0xffffu : 0u;     // TRUNC: no difference term
         auto entry = [&](int i) -> unsigned {
-            const unsigned w0 = (v[i >> 1] >> (16 * (i & 1))) & 0xffffu;
-            const unsigned w1 = i < 7 ? ((v[(i + 1) >> 1] >> (16 * ((i + 1) & 1))) & 0xffffu) : nx;
-            const unsigned dd = (unsigned)__half_as_ushort(
-                __hsub(__ushort_as_half((unsigned short)w1), __ushort_as_half((unsigned short)w0)));
-            return w0 | ((dd & lm) << 16);
+            const unsigned pr = (i & 1) ? __builtin_amdgcn_alignbit(i < 7 ? v[(i + 1) >> 1] : nx,
+                                                                    v[i >> 1], 16)
+                                        : v[i >> 1];
+            const h2v uu = __builtin_bit_cast(h2v, pr) * half2;
+            const unsigned dd = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)(uu.y - uu.x));
+            return (unsigned)__builtin_bit_cast(unsigned short, uu.x) | ((dd & lm) << 16);
         };
         // whole waves of interior chunks on 16-byte aligned entries (the
         // alignment is the same for every chunk of a row): no range tests
@@ -328,10 +339,11 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             if ((tid % TB) == 0) {
                 const int shift = (tid & 63) - ((tid & 63) % TB);
                 s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
-                // half pairs hold the stored values: the trace's scale
+                // half pairs hold half the stored values: twice the trace's
+                // scale (exact) restores them in the accumulate
                 if (HP) {
                     const int k = q < s1 ? (a.all_traces ? (int)q : a.list[q]) : 0;
-                    s_mask[kGMax + g] = __float_as_uint(q < s1 ? a.scale[k] : 0.f);
+                    s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[k] : 0.f);
                 }
             }
         }

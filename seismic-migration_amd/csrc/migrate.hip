@@ -20,9 +20,11 @@
 //             traces (sample n already times sqrt(n dt)) accumulate d alone
 //             (GN instantiation: one v_sqrt_f32 fewer) and end with tau.
 //             Gained FP16 traces (HP) stage half pairs instead, one dword
-//             (h[n], fp16(h[n+1]-h[n])) per entry of the stored values: half
-//             the LDS per row, twice the traces per stage, interpolation by
-//             v_fma_mix_f32 and the trace's scale applied in the accumulate.
+//             (h[n]/2, fp16(h[n+1]/2-h[n]/2)) per entry of the stored values
+//             (halved so that no difference of finite FP16 values
+//             overflows): half the LDS per row, twice the traces per stage,
+//             interpolation by v_fma_mix_f32 and twice the trace's scale
+//             applied in the accumulate.
 // Out-of-range samples need no test: the pair table's entries outside the
 // valid range of the interpolation rule are zero (TRUNC: 0 < n < ns,
 // src/psktmCPU.c:82; LINEAR: 0 <= n, n+1 < ns), and x is clamped to ns unless

@@ -15,9 +15,10 @@
  * used as is when the data directory is the reference's ../data),
  * fileSizeFourBytes, binReadUints + ktmIBM32toIEEE or binReadFloats, midpoint
  * and half offset from source/receiver (scripts/ktmConvertXYtoOffMid.m:12),
- * ktmMigrationGPU, fileWrite.
+ * ktmMigrationGPU (or ktmMigrationCPU with --cpu-entry: the same image),
+ * fileWrite.
  *
- * usage: psktm_main [--data DIR] [--float] [--v0 V] [--out FILE]
+ * usage: psktm_main [--data DIR] [--float] [--v0 V] [--out FILE] [--cpu-entry]
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -47,15 +48,17 @@ static FILE *open_in(const char *dir, const char *name, int *flag)
 int main(int argc, char **argv)
 {
     const char *dir = "../data", *out = NULL;
-    int float_mode = 0;
+    int float_mode = 0, cpu_entry = 0;
     float v0 = 0.0f;
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--data") && i + 1 < argc) dir = argv[++i];
         else if (!strcmp(argv[i], "--float")) float_mode = 1;
         else if (!strcmp(argv[i], "--v0") && i + 1 < argc) v0 = (float)atof(argv[++i]);
         else if (!strcmp(argv[i], "--out") && i + 1 < argc) out = argv[++i];
+        else if (!strcmp(argv[i], "--cpu-entry")) cpu_entry = 1;
         else {
-            fprintf(stderr, "usage: %s [--data DIR] [--float] [--v0 V] [--out FILE]\n", argv[0]);
+            fprintf(stderr, "usage: %s [--data DIR] [--float] [--v0 V] [--out FILE] [--cpu-entry]\n",
+                    argv[0]);
             return 2;
         }
     }
@@ -125,7 +128,10 @@ int main(int argc, char **argv)
     const size_t nimg = (size_t)X.imagePnts * Y.imagePnts * Z.imagePnts;
     float *image = (float *)malloc(sizeof(float) * nimg);
     if (!image || !vel) return fail("out of host memory");
-    ktmMigrationGPU(&X, &Y, &Z, &job, mx, my, hx, hy, traces, vel, image);
+    if (cpu_entry)
+        ktmMigrationCPU(&X, &Y, &Z, &job, mx, my, hx, hy, traces, vel, image);
+    else
+        ktmMigrationGPU(&X, &Y, &Z, &job, mx, my, hx, hy, traces, vel, image);
     if (psktm_last_error()[0]) return fail("migration");
 
     if (out) snprintf(path, sizeof(path), "%s", out);

@@ -10,7 +10,7 @@ from ._lib import (BF16, F16, F32, INTERP_LINEAR, INTERP_TRUNC, MATH_FAST, MATH_
                    VEL_1D, VEL_3D, ImageGrid, JobParams, PsktmError, last_error, load)
 from .api import (Batch, Grid, Plan, Survey, demigrate, dot_f64, ibm32_to_ieee,
                   ibm32_to_ieee_device, ibm32_to_ieee_exact, init_three_axes,
-                  init_two_axes, ktm_migration_gpu, load_ktm_config, migrate,
+                  init_two_axes, ktm_migration_cpu, ktm_migration_gpu, load_ktm_config, migrate,
                   migrate_exact, quantize, rho_filter, trace_gain, offset_sort, offset_classes,
                   select_offset, gather_pad, upsample_traces, segy_text_header,
                   segy_reel_header, segy_open, segy_trace_headers, segy_decode, segy_load)
@@ -20,7 +20,7 @@ __all__ = [
     "VEL_1D", "VEL_3D", "ImageGrid", "JobParams", "PsktmError", "last_error", "load",
     "Batch", "Grid", "Plan", "Survey", "demigrate", "dot_f64", "ibm32_to_ieee",
     "ibm32_to_ieee_device", "ibm32_to_ieee_exact", "init_three_axes", "init_two_axes",
-    "ktm_migration_gpu", "load_ktm_config", "migrate", "migrate_exact", "quantize",
+    "ktm_migration_cpu", "ktm_migration_gpu", "load_ktm_config", "migrate", "migrate_exact", "quantize",
     "rho_filter", "trace_gain", "offset_sort", "offset_classes", "select_offset", "gather_pad",
     "upsample_traces", "segy_text_header", "segy_reel_header", "segy_open",
     "segy_trace_headers", "segy_decode", "segy_load",

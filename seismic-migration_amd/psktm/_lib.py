@@ -93,6 +93,8 @@ _SIGNATURES = {
     "loadKDMConfig": (None, []),
     "ktmMigrationGPU": (None, [_P(ImageGrid), _P(ImageGrid), _P(ImageGrid), _P(JobParams),
                                _p, _p, _p, _p, _p, _p, _p]),
+    "ktmMigrationCPU": (None, [_P(ImageGrid), _P(ImageGrid), _P(ImageGrid), _P(JobParams),
+                               _p, _p, _p, _p, _p, _p, _p]),
     # extended API
     "psktm_last_error": (ctypes.c_char_p, []),
     "psktm_version": (ctypes.c_char_p, []),

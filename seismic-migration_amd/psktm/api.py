@@ -505,6 +505,18 @@ def _host(a, dtype=np.float32):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+def _legacy_migration(entry, imageX, imageY, imageZ, config, arrays, image):
+    L = _lib.load()
+    arrs = [_host(a) for a in arrays]
+    n = imageX.imagePnts * imageY.imagePnts * imageZ.imagePnts
+    if image is None:
+        image = np.empty(n, dtype=np.float32)
+    getattr(L, entry)(ctypes.byref(imageX), ctypes.byref(imageY), ctypes.byref(imageZ),
+                      ctypes.byref(config), *[a.ctypes.data_as(_p) for a in arrs],
+                      image.ctypes.data_as(_p))
+    return image
+
+
 def ktm_migration_gpu(imageX: ImageGrid, imageY: ImageGrid, imageZ: ImageGrid,
                       config: JobParams, midX, midY, offX, offY, traces, slowness,
                       image=None):
@@ -512,15 +524,21 @@ def ktm_migration_gpu(imageX: ImageGrid, imageY: ImageGrid, imageZ: ImageGrid,
 
     Returns the image (zeroed when the call failed; see :func:`last_error`).
     """
-    L = _lib.load()
-    arrs = [_host(a) for a in (midX, midY, offX, offY, traces, slowness)]
-    n = imageX.imagePnts * imageY.imagePnts * imageZ.imagePnts
-    if image is None:
-        image = np.empty(n, dtype=np.float32)
-    L.ktmMigrationGPU(ctypes.byref(imageX), ctypes.byref(imageY), ctypes.byref(imageZ),
-                      ctypes.byref(config), *[a.ctypes.data_as(_p) for a in arrs],
-                      image.ctypes.data_as(_p))
-    return image
+    return _legacy_migration("ktmMigrationGPU", imageX, imageY, imageZ, config,
+                             (midX, midY, offX, offY, traces, slowness), image)
+
+
+def ktm_migration_cpu(imageX: ImageGrid, imageY: ImageGrid, imageZ: ImageGrid,
+                      config: JobParams, midX, midY, offX, offY, traces, slowness,
+                      image=None):
+    """ktmMigrationCPU (src/psktmCPU.c:13-23) on host arrays: the same
+    arguments and the image of the reference loop run on one thread,
+    computed on the GPU by the reference-exact kernel.
+
+    Returns the image (zeroed when the call failed; see :func:`last_error`).
+    """
+    return _legacy_migration("ktmMigrationCPU", imageX, imageY, imageZ, config,
+                             (midX, midY, offX, offY, traces, slowness), image)
 
 
 def init_three_axes(imageX: ImageGrid, imageY: ImageGrid, imageZ: ImageGrid):

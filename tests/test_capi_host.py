@@ -35,6 +35,7 @@ def header_functions():
 def test_header_parsed():
     names = header_functions()
     assert "ktmMigrationGPU" in names and "psktm_migrate" in names and "initThreeAxes" in names
+    assert "ktmMigrationCPU" in names
     assert len(names) >= 30
 
 
@@ -179,20 +180,24 @@ def _small_job():
     return X, Y, Z, J, z3, np.ones((3, 20), np.float32), np.full(16, 2000.0, np.float32)
 
 
-def test_ktm_migration_gpu_invalid_inputs_leave_zero_image():
+@pytest.mark.parametrize("entry,name", [(psktm.ktm_migration_gpu, "ktmMigrationGPU"),
+                                        (psktm.ktm_migration_cpu, "ktmMigrationCPU")])
+def test_ktm_migration_invalid_inputs_leave_zero_image(entry, name):
     X, Y, Z, J, z3, tr, v = _small_job()
     img = np.full(64, 7.0, np.float32)
     J.tracePts = 0
-    out = psktm.ktm_migration_gpu(X, Y, Z, J, z3, z3, z3, z3, tr, v, image=img)
+    out = entry(X, Y, Z, J, z3, z3, z3, z3, tr, v, image=img)
     assert not np.any(out) and "invalid job" in psktm.last_error()
+    assert psktm.last_error().startswith(name)
 
 
-def test_ktm_migration_gpu_without_device_fails_cleanly():
+@pytest.mark.parametrize("entry", [psktm.ktm_migration_gpu, psktm.ktm_migration_cpu])
+def test_ktm_migration_without_device_fails_cleanly(entry):
     import torch
     if torch.cuda.is_available():
         pytest.skip("a GPU is present; the no-device path is exercised on CPU hosts")
     X, Y, Z, J, z3, tr, v = _small_job()
     img = np.full(64, 7.0, np.float32)
-    out = psktm.ktm_migration_gpu(X, Y, Z, J, z3, z3, z3, z3, tr, v, image=img)
+    out = entry(X, Y, Z, J, z3, z3, z3, z3, tr, v, image=img)
     assert not np.any(out)
     assert "device" in psktm.last_error().lower()

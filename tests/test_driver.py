@@ -84,10 +84,13 @@ def test_without_gpu_reports_device(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("float_mode", [False, True])
-def test_driver_end_to_end_bit_exact(tmp_path, float_mode):
+@pytest.mark.parametrize("float_mode,cpu_entry", [(False, False), (True, False), (False, True)])
+def test_driver_end_to_end_bit_exact(tmp_path, float_mode, cpu_entry):
+    """The job through ktmMigrationGPU, or through ktmMigrationCPU
+    (--cpu-entry), gives the reference loop's image bit for bit."""
     dec, (mx, my, hx, hy), vel, grid = write_job(str(tmp_path), float_mode=float_mode)
-    args = ["--data", str(tmp_path)] + (["--float"] if float_mode else [])
+    args = ["--data", str(tmp_path)] + (["--float"] if float_mode else []) + \
+        (["--cpu-entry"] if cpu_entry else [])
     r = run(*args)
     assert r.returncode == 0, r.stderr
     img = np.fromfile(tmp_path / "outputData.bin", dtype=np.float32)

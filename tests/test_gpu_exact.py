@@ -39,12 +39,16 @@ def test_line_2d_bit_exact(cuda):
     assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32))
 
 
-def test_legacy_ktm_migration_gpu_bit_exact(cuda):
+@pytest.mark.parametrize("entry", ["ktm_migration_gpu", "ktm_migration_cpu"])
+def test_legacy_ktm_migration_bit_exact(cuda, entry):
+    """Both reference entry points (src/psktmCUDA.cu:211-221,
+    src/psktmCPU.c:13-23) through the C ABI on host arrays: the single-thread
+    reference loop's image bit for bit."""
     d = synth.line_2d(n_cmp=24, n_off=12, ns=600, nt=500)
     nx, x0, dx, ny, y0, dy, nt, t0, dtau = d.grid
     X, Y, Z = psktm.ImageGrid(nx, x0, dx), psktm.ImageGrid(ny, y0, dy), psktm.ImageGrid(nt, t0, dtau)
     J = psktm.JobParams(d.traces.shape[0], d.traces.shape[1], d.dt)
-    img = psktm.ktm_migration_gpu(X, Y, Z, J, d.mid_x, d.mid_y, d.off_x, d.off_y, d.traces, d.vel)
+    img = getattr(psktm, entry)(X, Y, Z, J, d.mid_x, d.mid_y, d.off_x, d.off_y, d.traces, d.vel)
     assert psktm.last_error() == ""
     ref = oracle.pstm_f32(oracle.Grid(*d.grid), d.mid_x, d.mid_y, d.off_x, d.off_y, d.traces,
                           d.vel, d.dt).reshape(-1)

@@ -464,3 +464,45 @@ def test_half_pairs_stage_sizes_and_padding_bit_identical(cuda):
     for im in imgs[1:]:
         assert torch.equal(im.view(torch.int32), imgs[0].view(torch.int32))
     assert imgs[0].abs().max() > 0
+
+
+def test_half_pairs_full_fp16_range(cuda):
+    """Half pairs on samples at the ends of the FP16 range. The quantiser maps
+    a trace alternating +1 / -0.9995 to +32768 / -32752, whose difference
+    (65520) rounds to inf in FP16; caller rows alternating +-65504 and rows of
+    full-range noise differ by up to 131008. Half pairs hold halved values and
+    differences, so every image stays finite and matches the FP64 gained
+    operator on the dequantised values (1e-3: the FP16 rounding of differences
+    as large as the samples themselves), and the FP32-pair form on the same
+    values (1e-4)."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=49, ntr=1200, nx=9, ny=7)
+    ntr, ns = tr.shape
+    rng = np.random.default_rng(49)
+    n = np.arange(ns)
+    tr2 = tr.copy()
+    alt = np.where(n % 2 == 0, 1.0, -0.9995)
+    tr2[::3] = (alt[None, :] * rng.uniform(0.5, 2.0, (len(tr2[::3]), 1))).astype(np.float32)
+    plan, t = gpu_plan(cuda, grid, mid, off, ns, dt, ap)
+    q, sc = psktm.quantize(t(tr2), torch.float16)
+    qa = q[::3].cpu().numpy().astype(np.float64)
+    assert qa.max() == 32768.0 and qa.min() == -32752.0
+    assert np.isinf(np.float16(qa[0, 1]) - np.float16(qa[0, 0]))
+    qh = q.clone()
+    qh[1::3] = torch.as_tensor(np.where(n % 2 == 0, 65504.0, -65504.0)
+                               .astype(np.float16)).to(cuda)
+    qh[2::3] = torch.as_tensor(rng.uniform(-65504, 65504, (len(range(2, ntr, 3)), ns))
+                               .astype(np.float16)).to(cuda)
+    for name, data in (("quantised", q), ("full range", qh)):
+        print(f"case half pairs at the FP16 range ends: {name}", flush=True)
+        img = psktm.migrate(plan, data, t(vel), scale=sc, gained=True).cpu().numpy()
+        assert np.isfinite(img).all(), name
+        deq = (data.float() * sc[:, None]).cpu().numpy()
+        img32 = psktm.migrate(plan, torch.as_tensor(deq).to(cuda), t(vel),
+                              gained=True).cpu().numpy()
+        ref = oracle.pstm_f64(oracle.Grid(*grid), mid[:, 0], mid[:, 1], off[:, 0], off[:, 1],
+                              deq, vel, dt, vel3d=True, aperture=ap, interp=oracle.LINEAR,
+                              gained=True)
+        e, e32 = rel_l2(img, ref), rel_l2(img32, ref)
+        print(f"{name}: half pairs {e:.3e}, FP32 pairs {e32:.3e} vs FP64 gained")
+        assert e <= 1e-3 and e32 <= 1e-4, (name, e, e32)

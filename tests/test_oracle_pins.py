@@ -392,3 +392,90 @@ def test_prep_restatements_closed_forms():
     r = np.arange(6.0)[None] * 2 + 1
     u = oracle.upsample_time(r, 2)
     assert u.shape == (1, 21) and np.allclose(u[0], 1 + 2 * np.arange(21) / 4.0)
+
+
+def _binary32_loop(grid, mid, off, traces, vel, dt, weight="ref", dist="ref", ratio="ref",
+                   contract=False):
+    """src/psktmCPU.c:42-87 evaluated with numpy binary32 scalars-as-arrays:
+    every operation rounded to float32 once, in the association of the C
+    expressions (C evaluates a * b * c as (a * b) * c), traces added in k order.
+    Vectorised over (i, j, l), where each image sample's sum stays in k order.
+    The keyword arguments select plausible re-associations, for the check that
+    the comparison can tell them apart."""
+    f = np.float32
+    X = f(grid.x0) + np.arange(grid.nx, dtype=f) * f(grid.dx)
+    Y = f(grid.y0) + np.arange(grid.ny, dtype=f) * f(grid.dy)
+    Z = f(grid.tau0) + np.arange(grid.nt, dtype=f) * f(grid.dtau)
+    tauS = Z * Z
+    vv = vel.astype(f)
+    sS = f(1.0) / (vv * vv)
+    dtf = f(dt)
+    ns = traces.shape[1]
+    img = np.zeros((grid.nx, grid.ny, grid.nt), f)
+    for k in range(traces.shape[0]):
+        mx, my, hx, hy = f(mid[k, 0]), f(mid[k, 1]), f(off[k, 0]), f(off[k, 1])
+        if dist == "ref":
+            ax, bx = (mx - X) - hx, (mx - X) + hx                   # :65-66
+            ay, by = (my - Y) - hy, (my - Y) + hy                   # :71-72
+        else:
+            ax, bx = mx - (X + hx), mx - (X - hx)
+            ay, by = my - (Y + hy), my - (Y - hy)
+        A = (ax * ax)[:, None] + (ay * ay)[None, :]
+        B = (bx * bx)[:, None] + (by * by)[None, :]
+        A, B = A[:, :, None], B[:, :, None]
+        if contract:          # tauS + A*sS as one fused rounding
+            ua = (A.astype(np.float64) * sS + tauS).astype(f)
+            ub = (B.astype(np.float64) * sS + tauS).astype(f)
+        else:
+            ua, ub = tauS + A * sS, tauS + B * sS
+        t = np.sqrt(ua)                                             # :78
+        t = t + np.sqrt(ub)                                         # :79
+        q = t / dtf if ratio == "ref" else t * (f(1.0) / dtf)       # :80
+        with np.errstate(invalid="ignore"):
+            n = np.where(np.isfinite(q) & (q < ns), q, 0).astype(np.int64)
+        ok = (n > 0) & (n < ns)                                     # :82
+        d = traces[k][np.clip(n, 0, ns - 1)]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            if weight == "ref":
+                w = (d * t) * np.sqrt(tauS / t)                     # :84-85
+            elif weight == "right":
+                w = d * (t * np.sqrt(tauS / t))
+            else:             # the algebraic identity tau * sqrt(t)
+                w = d * (Z * np.sqrt(t))
+        img = img + np.where(ok, w, f(0.0)).astype(f)
+    return img
+
+
+def test_binary32_operation_order_bit_exact():
+    """The oracle's binary32 loop is bit-identical to an independent numpy
+    restatement of src/psktmCPU.c:42-87 in the C expressions' association,
+    and each plausible re-association - the weight as d*(t*s), the distance as
+    mx-(x+hx), a fused tauS + A*sS, t*(1/dt) for t/dt, tau*sqrt(t) for
+    t*sqrt(tau^2/t) - changes the bits on the same inputs, so this comparison
+    pins the order and not only the value."""
+    rng = np.random.default_rng(77)
+
+    def case(ntr, nt):
+        grid = Grid(5, 100.0, 12.5, 3, 40.0, 25.0, nt, 0.0, 0.002)
+        ns = nt * 5 // 4
+        mid = rng.uniform(60, 180, (ntr, 2)).astype(np.float32)
+        off = rng.uniform(-150, 150, (ntr, 2)).astype(np.float32)
+        traces = rng.standard_normal((ntr, ns)).astype(np.float32)
+        vel = np.linspace(1500, 2400, grid.nt).astype(np.float32)
+        return grid, mid, off, traces, vel, np.float32(0.004)
+
+    small, large = case(7, 96), case(200, 400)
+    for grid, mid, off, traces, vel, dt in (small, large):
+        got = oracle.pstm_f32(grid, mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], traces, vel, dt)
+        ref = _binary32_loop(grid, mid, off, traces, vel, dt)
+        assert np.count_nonzero(ref) > 0.5 * ref.size
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    # the truncated index hides t*(1/dt) except next to sample boundaries:
+    # the larger case has such samples
+    for kw, c in (({"weight": "right"}, small), ({"weight": "identity"}, small),
+                  ({"dist": "alt"}, small), ({"contract": True}, small),
+                  ({"ratio": "recip"}, large)):
+        ref = _binary32_loop(*c)
+        alt = _binary32_loop(*c, **kw)
+        changed = np.count_nonzero(alt.view(np.uint32) != ref.view(np.uint32))
+        assert changed > 0, kw
