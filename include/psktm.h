@@ -439,9 +439,18 @@ psktm_status psktm_segy_load(const psktm_segy_layout *layout, const char *path,
                              float *traces, int *headers, void *stream);
 
 /* FP64 reductions for conjugate gradients, stream-ordered, no host sync:
- * result[0] = sum_i a[i]*b[i] accumulated in double. */
+ * result[0] = sum_i a[i]*b[i] accumulated in double. Workgroup sums meet in
+ * an FP64 atomic add, so the last bits depend on their order. */
 psktm_status psktm_dot_f64(const float *a, const float *b, size_t n,
                            double *result, void *stream);
+/* The same sum in a fixed order: at most PSKTM_DOT_WORKSPACE workgroups write
+ * one partial each into workspace[] (caller-owned device memory of
+ * PSKTM_DOT_WORKSPACE doubles), then one workgroup adds them in index order,
+ * so equal inputs give equal bits on every run. Calls sharing a workspace
+ * must not overlap. */
+#define PSKTM_DOT_WORKSPACE 1024
+psktm_status psktm_dot_f64_ws(const float *a, const float *b, size_t n,
+                              double *result, double *workspace, void *stream);
 /* y[i] += (float)(num[0]/den[0]) * sign * x[i], scalars read on the device. */
 psktm_status psktm_axpy_ratio(float *y, const float *x, size_t n,
                               const double *num, const double *den,

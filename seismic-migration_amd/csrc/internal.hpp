@@ -142,6 +142,13 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image,
                               const psktm_velocity &vel, float *traces_out,
                               int accumulate, hipStream_t stream);
 
+// FP64 dot product in a fixed order (cg.hip): at most kDotPartials
+// workgroups write one partial each into `partial`, then one workgroup adds
+// them into result[0].
+constexpr int kDotPartials = PSKTM_DOT_WORKSPACE;
+psktm_status launch_dot(const float *a, const float *b, size_t n, double *result,
+                        double *partial, hipStream_t stream);
+
 // Per-call slowness/tau tables for a plan (migrate.hip).
 psktm_status prepare_tables(psktm_plan *p, const psktm_velocity &vel,
                             hipStream_t stream);

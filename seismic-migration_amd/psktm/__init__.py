@@ -8,7 +8,7 @@ and streams only.
 """
 from ._lib import (BF16, F16, F32, INTERP_LINEAR, INTERP_TRUNC, MATH_FAST, MATH_IEEE,
                    VEL_1D, VEL_3D, ImageGrid, JobParams, PsktmError, last_error, load)
-from .api import (Batch, Grid, Plan, Survey, demigrate, dot_f64, ibm32_to_ieee,
+from .api import (Batch, Grid, Plan, Survey, demigrate, dot_f64, dot_workspace, ibm32_to_ieee,
                   ibm32_to_ieee_device, ibm32_to_ieee_exact, init_three_axes,
                   init_two_axes, ktm_migration_cpu, ktm_migration_gpu, load_ktm_config, migrate,
                   migrate_exact, quantize, rho_filter, trace_gain, offset_sort, offset_classes,
@@ -18,7 +18,7 @@ from .api import (Batch, Grid, Plan, Survey, demigrate, dot_f64, ibm32_to_ieee,
 __all__ = [
     "BF16", "F16", "F32", "INTERP_LINEAR", "INTERP_TRUNC", "MATH_FAST", "MATH_IEEE",
     "VEL_1D", "VEL_3D", "ImageGrid", "JobParams", "PsktmError", "last_error", "load",
-    "Batch", "Grid", "Plan", "Survey", "demigrate", "dot_f64", "ibm32_to_ieee",
+    "Batch", "Grid", "Plan", "Survey", "demigrate", "dot_f64", "dot_workspace", "ibm32_to_ieee",
     "ibm32_to_ieee_device", "ibm32_to_ieee_exact", "init_three_axes", "init_two_axes",
     "ktm_migration_cpu", "ktm_migration_gpu", "load_ktm_config", "migrate", "migrate_exact", "quantize",
     "rho_filter", "trace_gain", "offset_sort", "offset_classes", "select_offset", "gather_pad",

@@ -135,6 +135,7 @@ _SIGNATURES = {
     "psktm_segy_load": (_c_int, [_P(PsktmSegyLayout), ctypes.c_char_p, ctypes.c_longlong,
                                  ctypes.c_longlong, _c_int, _p, _p, _p]),
     "psktm_dot_f64": (_c_int, [_p, _p, _c_size, _p, _p]),
+    "psktm_dot_f64_ws": (_c_int, [_p, _p, _c_size, _p, _p, _p]),
     "psktm_axpy_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _c_float, _p]),
     "psktm_xpby_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _p]),
 }

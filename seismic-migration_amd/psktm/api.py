@@ -337,16 +337,34 @@ def quantize(traces, out_dtype, *, stream=None):
     return out, scale
 
 
-def dot_f64(a, b, out=None, *, stream=None):
-    """sum(a*b) of float32 device vectors accumulated in FP64 (device result)."""
+DOT_WORKSPACE = 1024      # doubles in a psktm_dot_f64_ws workspace
+
+
+def dot_workspace(device):
+    """Workspace for :func:`dot_f64` in the fixed summation order."""
+    torch = _torch()
+    return torch.empty((DOT_WORKSPACE,), dtype=torch.float64, device=device)
+
+
+def dot_f64(a, b, out=None, *, workspace=None, stream=None):
+    """sum(a*b) of float32 device vectors accumulated in FP64 (device result).
+
+    With ``workspace`` (:func:`dot_workspace`), psktm_dot_f64_ws: the partials
+    are added in a fixed order, so equal inputs give equal bits on every call.
+    Without it, psktm_dot_f64 (the workgroup sums meet in an FP64 atomic)."""
     torch = _torch()
     L = _lib.load()
     _need(a, torch.float32, "a")
     _need(b, torch.float32, "b", a.numel())
     if out is None:
         out = torch.empty((1,), dtype=torch.float64, device=a.device)
-    check(L.psktm_dot_f64(_p(_ptr(a)), _p(_ptr(b)), ctypes.c_size_t(a.numel()),
-                          _p(_ptr(out)), _stream(stream)))
+    if workspace is None:
+        check(L.psktm_dot_f64(_p(_ptr(a)), _p(_ptr(b)), ctypes.c_size_t(a.numel()),
+                              _p(_ptr(out)), _stream(stream)))
+    else:
+        _need(workspace, torch.float64, "workspace", DOT_WORKSPACE)
+        check(L.psktm_dot_f64_ws(_p(_ptr(a)), _p(_ptr(b)), ctypes.c_size_t(a.numel()),
+                                 _p(_ptr(out)), _p(_ptr(workspace)), _stream(stream)))
     return out
 
 

@@ -12,16 +12,17 @@ interpolation weights, aperture and amplitude weight.
 * :func:`cgls` - conjugate gradients on the normal equations,
   min ||A m - d||^2, with every inner product accumulated in FP64 on the GPU
   and the step lengths alpha = gamma/delta, beta = gamma'/gamma formed on the
-  device from those FP64 scalars (``psktm_dot_f64``, ``psktm_axpy_ratio``,
-  ``psktm_xpby_ratio``); an iteration never waits for the host, so it can be
-  captured in a HIP graph.
+  device from those FP64 scalars (``psktm_dot_f64_ws``, in a fixed order, so
+  equal vectors give equal scalars; ``psktm_axpy_ratio``,
+  ``psktm_xpby_ratio``); an iteration never waits for the host and allocates
+  nothing, so it can be captured in a HIP graph.
 """
 from __future__ import annotations
 
 import ctypes
 
 from . import _lib
-from .api import Plan, _need, _ptr, _stream, demigrate, dot_f64, migrate
+from .api import Plan, _need, _ptr, _stream, demigrate, dot_f64, dot_workspace, migrate
 
 _p = ctypes.c_void_p
 
@@ -89,22 +90,26 @@ class CGLS:
         self.s = migrate(plan, self.r, vel)
         self.p = self.s.clone()
         self.q = torch.empty_like(data)
-        self.gamma = dot_f64(self.s.reshape(-1), self.s.reshape(-1))
+        self.ws = dot_workspace(dev)
+        self.gamma = self._dot(self.s, self.s)
         self.gamma_new = torch.empty_like(self.gamma)
         self.delta = torch.empty_like(self.gamma)
-        self.rr = dot_f64(self.r.reshape(-1), self.r.reshape(-1))
+        self.rr = self._dot(self.r, self.r)
+
+    def _dot(self, a, b, out=None):
+        return dot_f64(a.reshape(-1), b.reshape(-1), out, workspace=self.ws)
 
     def step(self):
         plan, vel = self.plan, self.vel
         demigrate(plan, self.p, vel, self.q)                               # q = A p
-        dot_f64(self.q.reshape(-1), self.q.reshape(-1), self.delta)        # delta = <q,q>
+        self._dot(self.q, self.q, self.delta)                              # delta = <q,q>
         _axpy_ratio(self.m, self.p, self.gamma, self.delta, +1.0)          # m += alpha p
         _axpy_ratio(self.r, self.q, self.gamma, self.delta, -1.0)          # r -= alpha q
         migrate(plan, self.r, vel, self.s)                                 # s = A' r
-        dot_f64(self.s.reshape(-1), self.s.reshape(-1), self.gamma_new)    # gamma' = <s,s>
+        self._dot(self.s, self.s, self.gamma_new)                          # gamma' = <s,s>
         _xpby_ratio(self.p, self.s, self.gamma_new, self.gamma)            # p = s + beta p
         self.gamma, self.gamma_new = self.gamma_new, self.gamma
-        dot_f64(self.r.reshape(-1), self.r.reshape(-1), self.rr)           # ||r||^2
+        self._dot(self.r, self.r, self.rr)                                 # ||r||^2
 
 
 def cgls(plan: Plan, vel, data, iters: int = 20, m0=None, history: bool = True):

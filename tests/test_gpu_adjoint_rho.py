@@ -205,8 +205,27 @@ def test_fp64_dot_and_ratio_updates(cuda):
     g = torch.Generator(device="cpu").manual_seed(4)
     a = torch.randn(1_000_003, generator=g)
     b = torch.randn(1_000_003, generator=g)
-    r = psktm.dot_f64(a.to(cuda), b.to(cuda)).item()
+    ad, bd = a.to(cuda), b.to(cuda)
+    r = psktm.dot_f64(ad, bd).item()
     assert abs(r - float((a.double() * b.double()).sum())) <= 1e-12 * a.numel()
+    # psktm_dot_f64_ws, fixed summation order: the same bits on every call,
+    # also from another stream with its own workspace
+    ws = psktm.dot_workspace(cuda)
+    r_ws = psktm.dot_f64(ad, bd, workspace=ws).item()
+    assert abs(r_ws - float((a.double() * b.double()).sum())) <= 1e-12 * a.numel()
+    again = [psktm.dot_f64(ad, bd, workspace=ws).item() for _ in range(3)]
+    s2 = torch.cuda.Stream()
+    ws2 = psktm.dot_workspace(cuda)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s2):
+        out2 = psktm.dot_f64(ad, bd, workspace=ws2, stream=s2)
+    s2.synchronize()
+    assert all(v == r_ws for v in again + [out2.item()])
+    for n in (0, 1, 255, 257, 70001):
+        ref = float((a[:n].double() * b[:n].double()).sum())
+        for w in (None, ws):
+            v = psktm.dot_f64(ad[:n], bd[:n], workspace=w).item()
+            assert abs(v - ref) <= 1e-12 * max(n, 1), (n, w is None)
     y = a.clone().to(cuda)
     num = torch.tensor([3.0], dtype=torch.float64, device=cuda)
     den = torch.tensor([2.0], dtype=torch.float64, device=cuda)
