@@ -451,6 +451,16 @@ psktm_status psktm_dot_f64(const float *a, const float *b, size_t n,
 #define PSKTM_DOT_WORKSPACE 1024
 psktm_status psktm_dot_f64_ws(const float *a, const float *b, size_t n,
                               double *result, double *workspace, void *stream);
+/* No counterpart in the reference: psktm_dot_f64_ws for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc) and pass no workspace. Same kernels, same
+ * bits; the partials live in a static device array of PSKTM_DOT_WORKSPACE
+ * doubles (8 KiB) that each GPU holds until the process ends. Any n fits (the
+ * first pass uses at most PSKTM_DOT_WORKSPACE workgroups), so no size is
+ * refused. Calls no hipMalloc/hipFree. All calls on one GPU share the array:
+ * calls that overlap on one GPU race. */
+psktm_status psktm_dot_f64_noalloc(const float *a, const float *b, size_t n,
+                                   double *result, void *stream);
 /* y[i] += (float)(num[0]/den[0]) * sign * x[i], scalars read on the device. */
 psktm_status psktm_axpy_ratio(float *y, const float *x, size_t n,
                               const double *num, const double *den,

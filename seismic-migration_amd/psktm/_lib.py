@@ -136,6 +136,7 @@ _SIGNATURES = {
                                  ctypes.c_longlong, _c_int, _p, _p, _p]),
     "psktm_dot_f64": (_c_int, [_p, _p, _c_size, _p, _p]),
     "psktm_dot_f64_ws": (_c_int, [_p, _p, _c_size, _p, _p, _p]),
+    "psktm_dot_f64_noalloc": (_c_int, [_p, _p, _c_size, _p, _p]),
     "psktm_axpy_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _c_float, _p]),
     "psktm_xpby_ratio": (_c_int, [_p, _p, _c_size, _p, _p, _p]),
 }
