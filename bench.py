@@ -14,14 +14,17 @@ plan (midpoint binning and per-tile aperture lists), rho-filter the traces
 times sqrt(n dt)) fused in, quantise them to FP16 with per-trace FP32
 scales, and migrate (LINEAR interpolation, FAST math, FP32 accumulation,
 weight tau per update). The FP16 form with the per-update weight tau*sqrt(t)
-(no gain) and the plain FP32 form (FP32 traces, per-update weight) are timed
-the same way and reported beside the headline.
+(no gain), the plain FP32 form (FP32 traces, per-update weight) and the FP32
+form of the headline method (FP32 gained traces) are timed the same way and
+reported beside the headline, with the FP16/FP32 ratio of each method.
 
 value = updates / s of the headline form, where one update is one (trace,
 in-aperture bin, output sample) triple counted by the plan. Every form's
 image is compared with the FP64 oracle of the per-update weight (the
-reference's operator with LINEAR interpolation) on a central sub-volume of
-bins (rel-L2). The reference's CPU loop (the oracle's binary32
+reference's operator with LINEAR interpolation) on 48 bins scattered over the
+central 40 x 40-bin sub-volume, the edges and the corners (or on a central
+N x N sub-volume, --oracle-bins N) (rel-L2), and the headline form also with
+its own FP64 operator on its stored values. The reference's CPU loop (the oracle's binary32
 restatement of src/psktmCPU.c) is timed on a bounded sample of the same
 workload on the host cores. Only rank 0 of one GPU runs.
 """
@@ -73,13 +76,15 @@ def parse():
     ap.add_argument("--traces", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 comparison")
-    ap.add_argument("--cpu-traces", type=int, default=4000,
+    ap.add_argument("--cpu-traces", type=int, default=10000,
                     help="traces in the CPU-baseline sample")
-    ap.add_argument("--oracle-bins", type=int, default=4,
-                    help="side of the central sub-volume compared with FP64")
-    ap.add_argument("--forms", default="f16g,f16,f32",
+    ap.add_argument("--oracle-bins", type=int, default=0,
+                    help="0: compare 48 bins scattered over the central 40x40 sub-volume, the "
+                         "edges and the corners with FP64; N > 0: the central N x N bins")
+    ap.add_argument("--forms", default="f16g,f16,f32,f32g",
                     help="timed forms: f16g (FP16, gained traces; the headline), f16, "
-                         "bf16, f32 (FP32 plain), f32g; the first is the headline")
+                         "bf16, f32 (FP32 plain), f32g (FP32, gained traces); the first is "
+                         "the headline")
     ap.add_argument("--workload", default="pstm3d", choices=["pstm3d", "batched", "lsm", "line2d"],
                     help="pstm3d: the headline 3-D workload (default); batched: 64 offset-class "
                          "gathers in one persistent launch; lsm: least-squares 2-D line, 20 CG "
@@ -101,6 +106,23 @@ def calibrate_copy(nbytes):
     b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"calibrate_copy_bytes": n * 4}), flush=True)
+
+
+def scattered_bins(nx, ny, seed=20261205):
+    """48 image bins for the FP64 comparison: 24 in the central 40 x 40-bin
+    sub-volume (bins 80-119), the 4 corners, the 4 edge midpoints, 8 more on
+    the edges and 8 anywhere."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    c = nx // 2 - 20
+    bins = [(int(i), int(j)) for i, j in rng.integers(c, c + 40, (24, 2))]
+    bins += [(0, 0), (0, ny - 1), (nx - 1, 0), (nx - 1, ny - 1)]
+    bins += [(0, ny // 2), (nx - 1, ny // 2), (nx // 2, 0), (nx // 2, ny - 1)]
+    for k in range(8):
+        t = int(rng.integers(1, nx - 1))
+        bins.append([(0, t), (nx - 1, t), (t, 0), (t, ny - 1)][k % 4])
+    bins += [(int(i), int(j)) for i, j in zip(rng.integers(0, nx, 8), rng.integers(0, ny, 8))]
+    return bins
 
 
 def host_cpu():
@@ -204,34 +226,37 @@ def main():
         log(f"{form}: {results[form]['ms_per_step']:.1f} ms/step, migrate kernel "
             f"{results[form]['kernel_ms']:.1f} ms, {results[form]['rate'] / 1e9:.1f} G updates/s")
 
-    # ---- accuracy vs the FP64 oracle on a central sub-volume ---------------
+    # ---- accuracy vs the FP64 oracle ------------------------------------------
     acc = {}
+    acc_bins = None
     if not args.no_oracle:
         import oracle
         nb = args.oracle_bins
-        c0x, c0y = grid.nx // 2 - nb // 2, grid.ny // 2 - nb // 2
-        bins = (c0x, c0x + nb, c0y, c0y + nb)
+        if nb > 0:
+            c0x, c0y = grid.nx // 2 - nb // 2, grid.ny // 2 - nb // 2
+            bins = [(c0x + i, c0y + j) for i in range(nb) for j in range(nb)]
+            acc_bins = f"central {nb}x{nb} bins (x {c0x}-{c0x + nb - 1}, y {c0y}-{c0y + nb - 1})"
+        else:
+            bins = scattered_bins(grid.nx, grid.ny)
+            acc_bins = ("48 bins: 24 in the central 40x40 sub-volume (80-119), 4 corners, "
+                        "12 on the edges, 8 anywhere")
         f32_traces = psktm.rho_filter(s.traces, dw_us, 2)
-        # traces that can reach the sub-volume (midpoint within the aperture
-        # of its bin rectangle)
-        x_lo, x_hi = c0x * grid.dx, (c0x + nb - 1) * grid.dx
-        y_lo, y_hi = c0y * grid.dy, (c0y + nb - 1) * grid.dy
-        ex = torch.clamp(torch.maximum(x_lo - s.mid_x, s.mid_x - x_hi), min=0)
-        ey = torch.clamp(torch.maximum(y_lo - s.mid_y, s.mid_y - y_hi), min=0)
-        sel = torch.nonzero(ex * ex + ey * ey <= (s.aperture * 1.001) ** 2).squeeze(1)
-        host = lambda t: t.index_select(0, sel).cpu().numpy()
+        host = lambda t: t.cpu().numpy()
+        og = oracle.Grid(*s.grid)
+        okw = dict(vel3d=True, aperture=s.aperture, interp=oracle.LINEAR)
+        hmx, hmy, hox, hoy, hvel = (host(a) for a in (s.mid_x, s.mid_y, s.off_x, s.off_y, s.vel))
         t1 = time.time()
-        hmx, hmy, hox, hoy = host(s.mid_x), host(s.mid_y), host(s.off_x), host(s.off_y)
-        htr, hvel = host(f32_traces), s.vel.cpu().numpy()
-        ref = np.zeros((nb, nb, grid.nt))
-        rows = 4                      # a few bin rows per call, with progress lines
-        for r0 in range(0, nb, rows):
-            r1 = min(nb, r0 + rows)
-            ref[r0:r1] = oracle.pstm_f64(oracle.Grid(*s.grid), hmx, hmy, hox, hoy, htr, hvel, dt,
-                                         vel3d=True, aperture=s.aperture, interp=oracle.LINEAR,
-                                         bins=(c0x + r0, c0x + r1, c0y, c0y + nb))
-            log(f"FP64 oracle: {r1}/{nb} bin rows ({time.time() - t1:.0f} s)")
-        log(f"FP64 oracle on {nb}x{nb} bins x {grid.nt} ({len(sel)} traces) in {time.time() - t1:.1f} s")
+        htr = host(f32_traces)
+        ref = None
+        for c in range(0, len(bins), 64):        # progress lines on the large sub-volumes
+            part = oracle.pstm_f64_bins(og, hmx, hmy, hox, hoy, htr, hvel, dt,
+                                        bins[c:c + 64], **okw)
+            ref = part if ref is None else np.concatenate([ref, part])
+            log(f"FP64 oracle: {min(c + 64, len(bins))}/{len(bins)} bins ({time.time() - t1:.0f} s)")
+        del htr
+        log(f"FP64 oracle on {len(bins)} bins x {grid.nt} in {time.time() - t1:.1f} s")
+        bi = torch.as_tensor([b[0] for b in bins], device=dev)
+        bj = torch.as_tensor([b[1] for b in bins], device=dev)
         plan = psktm.Plan(grid, survey, aperture=s.aperture)
         for form in ("f16g", "f32g", "f32", "f16", "bf16"):
             gained = form.endswith("g")
@@ -240,13 +265,24 @@ def main():
                 img = psktm.migrate(plan, src, s.vel, gained=gained)
             else:
                 dtype = torch.float16 if FORM_DTYPES[form] == "f16" else torch.bfloat16
-                q, sc = psktm.trace_gain(f32_traces, dt, dtype) if gained else \
-                    psktm.quantize(f32_traces, dtype)
+                if form == "f16g":     # the headline path: rho filter with the fused gain
+                    q, sc = psktm.rho_filter(s.traces, dw_us, 2, out_dtype=dtype, gain=True)
+                else:
+                    q, sc = psktm.quantize(f32_traces, dtype)
                 img = psktm.migrate(plan, q, s.vel, scale=sc, gained=gained)
-            sub = img[bins[0]:bins[1], bins[2]:bins[3]].double().cpu().numpy()
+            sub = img[bi, bj].double().cpu().numpy()
             acc[form] = dict(rel_l2=float(np.linalg.norm(sub - ref) / np.linalg.norm(ref)),
                              max_abs=float(np.max(np.abs(sub - ref))),
                              max_abs_ref=float(np.max(np.abs(ref))))
+            if form == "f16g":
+                # its own FP64 operator (weight tau) on the dequantised stored values
+                deq = (q.float() * sc[:, None]).cpu().numpy()
+                own = oracle.pstm_f64_bins(og, hmx, hmy, hox, hoy, deq, hvel, dt, bins,
+                                           gained=True, **okw) if nb == 0 else None
+                del deq
+                if own is not None:
+                    acc[form]["rel_l2_vs_own_fp64_on_stored"] = float(
+                        np.linalg.norm(sub - own) / np.linalg.norm(own))
             log(f"{form}: rel-L2 vs FP64 {acc[form]['rel_l2']:.3e}")
         plan.close()
         del f32_traces
@@ -295,6 +331,8 @@ def main():
 
     f32 = results.get("f32")
     f16 = results.get("f16")
+    f32g = results.get("f32g")
+    f16g = results.get("f16g")
     kern_s = head["kernel_ms"] / 1e3
     per_update = VALU_OPS_PER_UPDATE_GAINED if head_form.endswith("g") else VALU_OPS_PER_UPDATE
     valu_ops = updates * per_update / kern_s
@@ -328,8 +366,11 @@ def main():
         "forms": {k: {"G_updates_per_s": v["rate"] / 1e9, "ms_per_step": v["ms_per_step"],
                       "migrate_kernel_ms": v["kernel_ms"]} for k, v in results.items()},
         "fp16_over_fp32": (head["rate"] / f32["rate"]) if f32 else None,
-        "fp16_over_fp32_same_weight": (f16["rate"] / f32["rate"]) if (f16 and f32) else None,
+        "fp16_over_fp32_same_method": {
+            "gained_traces": (f16g["rate"] / f32g["rate"]) if (f16g and f32g) else None,
+            "per_update_weight": (f16["rate"] / f32["rate"]) if (f16 and f32) else None},
         "accuracy_vs_fp64": acc or None,
+        "accuracy_bins": acc_bins,
         "roofline": {
             "bound": "valu",
             "achieved": valu_ops / 1e12,

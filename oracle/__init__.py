@@ -48,6 +48,10 @@ def lib():
                                       _p, _p, _p, _p, _p, _p, _p, _i, _f, _i, _i,
                                       _i, _i, _i, _i, _p, _i]
         L.oracle_pstm_f64.restype = None
+        L.oracle_pstm_f64_bins.argtypes = [_i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _i, _f,
+                                           _p, _p, _p, _p, _p, _p, _p, _i, _f, _i, _i,
+                                           _p, _p, _i, _p, _i]
+        L.oracle_pstm_f64_bins.restype = None
         L.oracle_pstm_adj_f64.argtypes = [_i, _f, _f, _i, _f, _f, _i, _f, _f, _i, _i, _f,
                                           _p, _p, _p, _p, _p, _p, _i, _f, _i, _p, _i]
         L.oracle_pstm_adj_f64.restype = None
@@ -148,6 +152,34 @@ def pstm_f64(grid: Grid, mid_x, mid_y, off_x, off_y, traces, vel, dt, *, vel3d=F
                           int(bool(vel3d)), float(aperture), int(interp), int(bool(gained)),
                           int(ix0), int(ix1), int(iy0), int(iy1),
                           out.ctypes.data_as(_p), int(threads))
+    return out
+
+
+def pstm_f64_bins(grid: Grid, mid_x, mid_y, off_x, off_y, traces, vel, dt, bins, *,
+                  vel3d=False, aperture=0.0, interp=LINEAR, scale=None, gained=False,
+                  threads=0):
+    """:func:`pstm_f64` on an explicit list of bins [(i, j), ...] -> [nb, nt]."""
+    traces = np.ascontiguousarray(traces, dtype=np.float32)
+    ntr, ns = traces.shape
+    bi = np.ascontiguousarray([b[0] for b in bins], dtype=np.int32)
+    bj = np.ascontiguousarray([b[1] for b in bins], dtype=np.int32)
+    assert bi.min() >= 0 and bi.max() < grid.nx and bj.min() >= 0 and bj.max() < grid.ny
+    out = np.zeros((len(bins), grid.nt), dtype=np.float64)
+    keep, ptrs = [], []
+    for a in (mid_x, mid_y, off_x, off_y, traces):
+        arr, p = _c(a, np.float32)
+        keep.append(arr)
+        ptrs.append(p)
+    if scale is not None:
+        sc, sp = _c(scale, np.float32)
+        keep.append(sc)
+    else:
+        sp = None
+    va, vp = _c(vel, np.float32)
+    lib().oracle_pstm_f64_bins(*grid.args(), ntr, ns, float(dt), *ptrs, sp, vp,
+                               int(bool(vel3d)), float(aperture), int(interp), int(bool(gained)),
+                               bi.ctypes.data_as(_p), bj.ctypes.data_as(_p), len(bins),
+                               out.ctypes.data_as(_p), int(threads))
     return out
 
 

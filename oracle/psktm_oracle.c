@@ -281,6 +281,65 @@ static double *slowness_f64(int nx, int ny, int nt, const float *vel, int vel3d)
  * on which (trace, bin) pairs exist. traces may be scaled per trace by
  * scale[k] (NULL = 1).
  */
+/* One image column (bin i, j) of oracle_pstm_f64 into o[nt]. */
+static void pstm_f64_column(int i, int j, int ny, int nt, const float *xx, const float *yy,
+                            const float *zz, const double *s, int vel3d, int use_ap, float r2,
+                            int ntr, int ns, float dt, const float *midX, const float *midY,
+                            const float *offX, const float *offY, const float *traces,
+                            const float *scale, int interp, int gained, double *o)
+{
+    const double x = xx[i], y = yy[j];
+    const double *sb = vel3d ? s + ((size_t)i * ny + j) * nt : s;
+    for (int k = 0; k < ntr; k++) {
+        if (use_ap && !in_aperture(midX[k], midY[k], xx[i], yy[j], r2)) continue;
+        const double ex = (double)midX[k] - x, ey = (double)midY[k] - y;
+        const double hx = offX[k], hy = offY[k];
+        const double A = (ex - hx) * (ex - hx) + (ey - hy) * (ey - hy);
+        const double B = (ex + hx) * (ex + hx) + (ey + hy) * (ey + hy);
+        const float *tr = traces + (size_t)k * ns;
+        const double sc = scale ? (double)scale[k] : 1.0;
+        for (int l = 0; l < nt; l++) {
+            long n; double w0, w1;
+            if (!update_f64(zz[l], sb[l], A, B, dt, ns, interp, gained, &n, &w0, &w1))
+                continue;
+            double d = w0 * (double)tr[n];
+            if (w1 != 0.0) d += w1 * (double)tr[n + 1];
+            o[l] += sc * d;
+        }
+    }
+}
+
+/* oracle_pstm_f64 on an explicit list of bins (bin_i[b], bin_j[b]),
+ * out[b][nt]: scattered columns (edges, corners, interior) in one parallel
+ * call. */
+void oracle_pstm_f64_bins(int nx, float x0, float dx, int ny, float y0, float dy,
+                          int nt, float tau0, float dtau, int ntr, int ns, float dt,
+                          const float *midX, const float *midY, const float *offX,
+                          const float *offY, const float *traces, const float *scale,
+                          const float *vel, int vel3d, float aperture, int interp,
+                          int gained, const int *bin_i, const int *bin_j, int nbins,
+                          double *out, int nthreads)
+{
+    float *xx = (float *)malloc(sizeof(float) * (size_t)nx);
+    float *yy = (float *)malloc(sizeof(float) * (size_t)ny);
+    float *zz = (float *)malloc(sizeof(float) * (size_t)nt);
+    axis_f32(nx, x0, dx, xx);
+    axis_f32(ny, y0, dy, yy);
+    axis_f32(nt, tau0, dtau, zz);
+    double *s = slowness_f64(nx, ny, nt, vel, vel3d);
+    const int use_ap = aperture > 0.0f;
+    const float r2 = aperture * aperture;
+    memset(out, 0, sizeof(double) * (size_t)nbins * nt);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbins; b++)
+        pstm_f64_column(bin_i[b], bin_j[b], ny, nt, xx, yy, zz, s, vel3d, use_ap, r2, ntr, ns,
+                        dt, midX, midY, offX, offY, traces, scale, interp, gained,
+                        out + (size_t)b * nt);
+    free(xx); free(yy); free(zz); free(s);
+}
+
 void oracle_pstm_f64(int nx, float x0, float dx, int ny, float y0, float dy,
                      int nt, float tau0, float dtau, int ntr, int ns, float dt,
                      const float *midX, const float *midY, const float *offX,
@@ -304,29 +363,10 @@ void oracle_pstm_f64(int nx, float x0, float dx, int ny, float y0, float dy,
     if (nthreads > 0) omp_set_num_threads(nthreads);
 
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int b = 0; b < nbins; b++) {
-        const int i = ix0 + b / sy, j = iy0 + b % sy;
-        const double x = xx[i], y = yy[j];
-        double *o = out + (size_t)b * nt;
-        const double *sb = vel3d ? s + ((size_t)i * ny + j) * nt : s;
-        for (int k = 0; k < ntr; k++) {
-            if (use_ap && !in_aperture(midX[k], midY[k], xx[i], yy[j], r2)) continue;
-            const double ex = (double)midX[k] - x, ey = (double)midY[k] - y;
-            const double hx = offX[k], hy = offY[k];
-            const double A = (ex - hx) * (ex - hx) + (ey - hy) * (ey - hy);
-            const double B = (ex + hx) * (ex + hx) + (ey + hy) * (ey + hy);
-            const float *tr = traces + (size_t)k * ns;
-            const double sc = scale ? (double)scale[k] : 1.0;
-            for (int l = 0; l < nt; l++) {
-                long n; double w0, w1;
-                if (!update_f64(zz[l], sb[l], A, B, dt, ns, interp, gained, &n, &w0, &w1))
-                    continue;
-                double d = w0 * (double)tr[n];
-                if (w1 != 0.0) d += w1 * (double)tr[n + 1];
-                o[l] += sc * d;
-            }
-        }
-    }
+    for (int b = 0; b < nbins; b++)
+        pstm_f64_column(ix0 + b / sy, iy0 + b % sy, ny, nt, xx, yy, zz, s, vel3d, use_ap, r2,
+                        ntr, ns, dt, midX, midY, offX, offY, traces, scale, interp, gained,
+                        out + (size_t)b * nt);
     free(xx); free(yy); free(zz); free(s);
 }
 
