@@ -85,11 +85,13 @@ def parse():
                     help="timed forms: f16g (FP16, gained traces; the headline), f16, "
                          "bf16, f32 (FP32 plain), f32g (FP32, gained traces); the first is "
                          "the headline")
-    ap.add_argument("--workload", default="pstm3d", choices=["pstm3d", "batched", "lsm", "line2d"],
+    ap.add_argument("--workload", default="pstm3d",
+                    choices=["pstm3d", "batched", "lsm", "line2d", "rates"],
                     help="pstm3d: the headline 3-D workload (default); batched: 64 offset-class "
                          "gathers in one persistent launch; lsm: least-squares 2-D line, 20 CG "
                          "iterations; line2d: the 2-D line, reference-exact GPU form vs the "
-                         "reference CPU loop")
+                         "reference CPU loop; rates: gfx950 VALU, v_sqrt_f32, LDS gather and "
+                         "LDS atomic rates behind the roofline figures")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache (batched workload)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (scales the "
@@ -148,7 +150,8 @@ def main():
         if args.gpus != 1:
             print(json.dumps({"error": "one GPU only"}))
             sys.exit(2)
-        {"batched": bench_batched, "lsm": bench_lsm, "line2d": bench_line2d}[args.workload](args)
+        {"batched": bench_batched, "lsm": bench_lsm, "line2d": bench_line2d,
+         "rates": bench_rates}[args.workload](args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -661,6 +664,79 @@ def bench_line2d(args):
                          "sample": "full 2-D line, best of 3, race-free restatement"},
         "cpu_racy_as_written_G_updates_per_s": updates / racy / 1e9,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_rates(args):
+    """gfx950 microbenchmarks (seismic-migration_amd/csrc/rates/rates.hip):
+    8 independent FMA or v_sqrt_f32 chains per lane, v_sqrt_f32 mixed with
+    FMAs, random ds_read_b64 gathers, and LDS atomic adds (FP32, FP64, u32)
+    with lane-consecutive, 4-apart (the adjoint's lanes own 4 consecutive tau
+    samples) and random slots. 2,048 workgroups of 256 lanes (8 waves per
+    SIMD); the time is the second of two launches."""
+    import ctypes
+
+    import torch
+    torch.cuda.init()
+    lib = ctypes.CDLL(os.path.join(ROOT, "seismic-migration_amd", "psktm", "librates.so"))
+    lib.psktm_rates_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_float)]
+    blocks, lanes = 2048, 2048 * 256
+
+    def run(kind, iters):
+        ms = ctypes.c_float(0)
+        rc = lib.psktm_rates_run(kind, blocks, iters, ctypes.byref(ms))
+        if rc:
+            raise RuntimeError(f"rates kind {kind}: rc {rc}")
+        return ms.value / 1e3
+
+    res = {}
+    t = run(0, 65536)
+    res["v_fma_f32_lane_ops_per_s"] = lanes * 8 * 65536 / t
+    for kind, name in ((5, "v_fma_f32_vgpr"), (6, "v_add_f32_vgpr"), (7, "v_mul_f32_vgpr")):
+        for iters in (4096, 65536):
+            res[f"{name}_iters{iters}_lane_ops_per_s"] = lanes * 8 * iters / run(kind, iters)
+    for kind, name, fmas in ((1, "sqrt_alone", 0), (2, "sqrt_with_8_fma", 8),
+                             (3, "sqrt_with_64_fma", 64)):
+        t = run(kind, 2048)
+        res[f"v_sqrt_f32_{name}_per_s"] = lanes * 8 * 2048 / t
+        # issue slots: FMA-equivalent time of 8 sqrt (+ their v_add) and the FMAs
+        res[f"{name}_fma_slots_per_8_sqrt"] = (t / (lanes * 2048)) * \
+            res["v_fma_f32_lane_ops_per_s"] - fmas
+    t = run(4, 2048)
+    res["ds_read_b64_random_per_s"] = lanes * 8 * 2048 / t
+    res["ds_read_b64_random_TBps"] = res["ds_read_b64_random_per_s"] * 8 / 1e12
+    for ty, tname in ((1, "f32"), (2, "f64"), (3, "u32")):
+        for p, pname in ((0, "consecutive"), (1, "stride4"), (2, "random")):
+            t = run(10 * ty + p, 512)
+            res[f"ds_add_{tname}_{pname}_per_s"] = lanes * 8 * 512 / t
+            log(f"ds_add_{tname} {pname}: {res[f'ds_add_{tname}_{pname}_per_s']:.3e}/s")
+    # issue rates of the migration loop's instruction types at 2-8 resident
+    # waves per SIMD (dynamic LDS per 256-lane workgroup limits residency)
+    lib.psktm_rates_issue.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_float)]
+    ops = ["v_add_f32", "v_fma_f32", "v_cvt_i32_f32+v_cvt_f32_i32", "v_fract_f32",
+           "v_lshl_add_u32", "v_fma_mix_f32", "v_fma_f32_sgpr", "v_add_f32+v_sqrt_f32"]
+    issue = {}
+    for op, name in enumerate(ops):
+        for w in (2, 4, 6, 8):
+            ms = ctypes.c_float(0)
+            iters = 4096
+            rc = lib.psktm_rates_issue(op, 256 * w * 4, iters, 160 * 1024 // w - 1024,
+                                       ctypes.byref(ms))
+            if rc:
+                raise RuntimeError(f"issue probe {name}: rc {rc}")
+            per = 16 if op == 2 else 8            # two conversions per chain step
+            issue[f"{name}@{w}w"] = 256 * w * 4 * 256 * per * iters / (ms.value / 1e3)
+            log(f"issue {name} at {w} waves/SIMD: {issue[f'{name}@{w}w']:.3e} lane-instr/s")
+    res["issue_lane_instr_per_s"] = issue
+    nominal = 256 * 64 * 2.4e9
+    line = {"metric": "gfx950 rates (VALU, v_sqrt_f32, LDS gathers and atomics)",
+            "value": res["v_fma_f32_lane_ops_per_s"] / nominal, "unit":
+            "v_fma_f32 lane-ops/s over 256 CU x 64 lanes x 2.4 GHz", "n_gpus": 1,
+            "higher_is_better": True, "data": "synthetic",
+            "config": {"workload": "microbenchmarks, 2,048 workgroups x 256 lanes"},
+            "rates": res}
     print(json.dumps(line), flush=True)
 
 

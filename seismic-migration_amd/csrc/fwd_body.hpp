@@ -292,9 +292,13 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
     // that stays inside the padded rows (with a margin far above the FP32
     // rounding of A, B and v_sqrt_f32), the index needs no clamp: samples
     // at and past ns read zero pairs either way, so the image is the same.
+    // Every comparison is written so that a NaN (from a NaN coordinate, a
+    // NaN velocity, or inf * 0) fails it, so the unclamped loop only ever
+    // sees finite times inside the rows (fmaxf would drop a NaN operand).
     bool unclamped = false;
     {
         float xb = 0.f;
+        bool finite = true;
         if (a.lim > a.ns) {
             const float r = *a.reach;
             const float r2 = r * r;
@@ -303,12 +307,18 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                 float sm = sv[0][j];
                 if (V3) {
 #pragma unroll
-                    for (int b = 1; b < TB; ++b) sm = fmaxf(sm, sv[V3 ? b : 0][j]);
+                    for (int b = 1; b < TB; ++b) {
+                        const float v = sv[V3 ? b : 0][j];
+                        finite = finite && (v == v);
+                        sm = v > sm ? v : sm;
+                    }
                 }
-                xb = fmaxf(xb, sqrtf(fmaf(r2, sm, tq[j])));
+                const float v = sqrtf(fmaf(r2, sm, tq[j]));
+                finite = finite && (v == v);
+                xb = v > xb ? v : xb;
             }
         }
-        const bool ok = a.lim > a.ns && 2.f * xb * 1.0001f + 2.f < (float)a.lim;
+        const bool ok = finite && a.lim > a.ns && 2.f * xb * 1.0001f + 2.f < (float)a.lim;
         unclamped = !__syncthreads_or(!ok);
     }
 
