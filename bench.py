@@ -670,7 +670,7 @@ def bench_line2d(args):
 def bench_rates(args):
     """gfx950 microbenchmarks (seismic-migration_amd/csrc/rates/rates.hip):
     8 independent FMA or v_sqrt_f32 chains per lane, v_sqrt_f32 mixed with
-    FMAs, random ds_read_b64 gathers, and LDS atomic adds (FP32, FP64, u32)
+    FMAs, random ds_read_b64 gathers, and LDS atomic adds (FP32, FP64, u32, u64)
     with lane-consecutive, 4-apart (the adjoint's lanes own 4 consecutive tau
     samples) and random slots. 2,048 workgroups of 256 lanes (8 waves per
     SIMD); the time is the second of two launches."""
@@ -706,7 +706,7 @@ def bench_rates(args):
     t = run(4, 2048)
     res["ds_read_b64_random_per_s"] = lanes * 8 * 2048 / t
     res["ds_read_b64_random_TBps"] = res["ds_read_b64_random_per_s"] * 8 / 1e12
-    for ty, tname in ((1, "f32"), (2, "f64"), (3, "u32")):
+    for ty, tname in ((1, "f32"), (2, "f64"), (3, "u32"), (4, "u64")):
         for p, pname in ((0, "consecutive"), (1, "stride4"), (2, "random")):
             t = run(10 * ty + p, 512)
             res[f"ds_add_{tname}_{pname}_per_s"] = lanes * 8 * 512 / t

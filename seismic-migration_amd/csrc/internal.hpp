@@ -84,7 +84,7 @@ __device__ __forceinline__ float axis_f32(float start, float step, int i)
 } // namespace psktm
 
 // A migration plan: geometry binned into output tiles (see psktm_plan_create).
-// Doubles per adjoint accumulator row for traces of ns samples: slots for
+// 64-bit slots per adjoint accumulator row for traces of ns samples: slots for
 // samples 0 .. ns+1 (the last two take discarded contributions) with one pad
 // slot every 16 samples.
 inline int adj_row(int ns) { return (ns + 2) + ((ns + 2) >> 4) + 1; }
@@ -128,6 +128,7 @@ struct psktm_plan {
     size_t tsq_cap = 0;
     float *partial = nullptr;       // split-K partial images [splits][image]
     size_t partial_cap = 0;
+    unsigned *adj_wmax = nullptr;   // [1] adjoint: max |image * tau * sqrt(dt)| (bits)
 
     size_t bytes = 0;
 };

@@ -33,10 +33,12 @@
 // ops per update); HBM traffic is the trace rows, read once per tile.
 //
 // Adjoint (scatter, trace-stationary). A workgroup owns a batch of traces
-// from one midpoint cell, accumulates them in LDS with ds_add_f64, visits
-// every tile within reach of the cell and writes each trace once. It uses the
-// same traveltime, clamp, interpolation weights and aperture bits as the
-// forward, so the pair is an exact adjoint up to FP32 rounding.
+// from one midpoint cell, visits every tile within reach of the cell and
+// writes each trace once. Lanes merge their runs of equal trace samples in
+// FP32 registers and add them to LDS as exact 64-bit fixed-point integers
+// (ds_add_u64; no FP64, and the sums do not depend on the atomics' order). It
+// uses the same traveltime, validity rule, interpolation weights and aperture
+// bits as the forward, so the pair is an exact adjoint up to FP32 rounding.
 #include "fwd_body.hpp"
 
 #include <algorithm>
@@ -58,6 +60,8 @@ struct AdjArgs {
     int tile_x, tile_y, tiles_x, tiles_y, n_chunks;
     int cells_y, cell_mx, cell_my, win_x, win_y;
     float nsf, r2, wscale;
+    float bound;                  // bound on (sum of |contributions| to one slot) / max|image*w|
+    const unsigned *wmax;         // max |image * tau * sqrt(dt)| (bits), per launch
     int use_ap, interp, nsa, batch;
     const float4 *geo;
     const int *sorted_ids;
@@ -82,7 +86,52 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
                                           split, s_pairs, s_ab, s_mask);
 }
 
-template <int NTH, int TB, int TL, bool V3, typename Acc>
+// Exact fixed-point accumulation of the adjoint. A contribution y, already
+// multiplied by 2^S, is added as the 64-bit two's complement integer
+// rint(y); S is chosen per launch (from adj_max_kernel) so that no sum can
+// reach 2^53 in magnitude, so every rint(y) has |rint(y)| < 2^54 and the
+// integer sums never overflow. Integer addition is associative: the adjoint's
+// output does not depend on the order of the LDS atomics. No FP64 is used.
+__device__ __forceinline__ unsigned long long to_fixed(float y)
+{
+#pragma clang fp contract(off)
+    const float yi = rintf(y);                                  // |yi| < 2^54
+    const float fh = floorf(yi * 5.9604644775390625e-08f);      // floor(yi / 2^24)
+    const unsigned lo = (unsigned)fmaf(-fh, 16777216.0f, yi);   // yi - fh 2^24, in [0, 2^24)
+    const int hi = (int)fh;                                     // |hi| < 2^30
+    const unsigned w0 = ((unsigned)hi << 24) | lo;
+    const unsigned w1 = (unsigned)(hi >> 8);                    // sign extension
+    return ((unsigned long long)w1 << 32) | w0;
+}
+
+// The float nearest (within an ulp) to a fixed-point sum u (|u| < 2^53).
+__device__ __forceinline__ float from_fixed(unsigned long long u)
+{
+    const int hi = (int)(unsigned)(u >> 32);                    // |hi| < 2^21: exact
+    return fmaf((float)hi, 4294967296.0f, (float)(unsigned)u);
+}
+
+// Per launch: m = max over the image of |image * tau * sqrt(dt)| (order-free
+// atomic max of the bit patterns; any non-finite sample gives a NaN pattern
+// above every finite one).
+__global__ void adj_max_kernel(const float *__restrict__ image, long long n, int nt, float tau0,
+                               float dtau, float wscale, unsigned *__restrict__ out)
+{
+    float m = 0.f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int l = (int)(i % nt);
+        const float v = fabsf(image[i] * (axis_f32(tau0, dtau, l) * wscale));
+        m = (v > m || v != v) ? v : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float u = __shfl_xor(m, o);
+        m = (u > m || u != u) ? u : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m != m ? 0x7fc00000u : __float_as_uint(m));
+}
+
+template <int NTH, int TB, int TL, bool V3, bool LIN>
 __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
 {
     // Products and sums round one at a time (explicit fmaf calls stay
@@ -90,7 +139,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     // schedules the merged accumulation below.
 #pragma clang fp contract(off)
     extern __shared__ __align__(8) unsigned char s_raw[];
-    Acc *s_acc = reinterpret_cast<Acc *>(s_raw);
+    unsigned long long *s_acc = reinterpret_cast<unsigned long long *>(s_raw);
     __shared__ float2 s_ab[kAdjMax][TB];
     __shared__ unsigned s_mask[kAdjMax];
     __shared__ int s_k[kAdjMax];
@@ -99,16 +148,30 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const int c = a.items[2 * blockIdx.x], p0 = a.items[2 * blockIdx.x + 1];
     const int p1 = min(p0 + a.batch, a.cell_start[c + 1]);
     const int ng = p1 - p0;
-    for (int q = tid; q < a.batch * a.nsa; q += NTH) s_acc[q] = Acc(0);
+    for (int q = tid; q < a.batch * a.nsa; q += NTH) s_acc[q] = 0ull;
     if (tid < kAdjMax) s_k[tid] = (tid < ng) ? a.sorted_ids[p0 + tid] : -1;
+
+    // fixed-point scale 2^S: every sum stays below 2^53 (see to_fixed).
+    // mx * bound < 2^(e1 + e2) bounds the sum of |contributions| to a slot.
+    const float mx = __uint_as_float(*a.wmax);
+    const bool bad = !(mx <= 3.0e38f);           // NaN or inf in the image
+    int e1 = 0, e2 = 0;
+    (void)frexpf(bad ? 0.f : mx, &e1);
+    (void)frexpf(a.bound, &e2);
+    // 2^S as two float factors: S reaches 250 for images near the FP32
+    // denormal range (and is negative for images near FLT_MAX)
+    const int S = (bad || !(mx > 0.f)) ? 0 : min(250, max(-120, 53 - (e1 + e2)));
+    const int Sa = min(S, 120), Sb = S - Sa;
+    const float scale_a = ldexpf(1.f, Sa), scale_b = ldexpf(1.f, Sb);
     __syncthreads();
 
     const int CI = c / a.cells_y, CJ = c % a.cells_y;
     const int ti0 = max(0, CI - a.cell_mx - a.win_x), ti1 = min(a.tiles_x - 1, CI - a.cell_mx + a.win_x);
     const int tj0 = max(0, CJ - a.cell_my - a.win_y), tj1 = min(a.tiles_y - 1, CJ - a.cell_my + a.win_y);
-    const int lo = (a.interp == PSKTM_INTERP_LINEAR) ? 0 : 1;
-    const int hi = (a.interp == PSKTM_INTERP_LINEAR) ? a.ns - 2 : a.ns - 1;
-    const bool linear = a.interp == PSKTM_INTERP_LINEAR;
+    // valid sample index: LINEAR 0 <= n <= ns-2, TRUNC 1 <= n <= ns-1
+    // (src/psktmCPU.c:82); anything else goes to the discarded slots ns, ns+1
+    const float xhi = LIN ? a.nsf - 1.f : a.nsf;
+    const float xlo = LIN ? 0.f : 1.f;
 
     for (int TI = ti0; TI <= ti1; ++TI) {
         for (int TJ = tj0; TJ <= tj1; ++TJ) {
@@ -137,11 +200,15 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                     // each lane takes TL consecutive tau samples, so that the
                     // lanes of a wave scatter to samples further apart
                     const int lbase = ch * (NTH * TL) + tid * TL;
-                    float tq[TL];
+                    float tq[TL], wt[TL], sl1[TL];
 #pragma unroll
                     for (int j = 0; j < TL; ++j) {
                         const int l = lbase + j;
-                        tq[j] = (l < a.nt) ? a.tsq[l] : 0.f;
+                        const bool ok = l < a.nt;
+                        tq[j] = ok ? a.tsq[l] : 0.f;
+                        // weight tau*sqrt(dt) (the forward's epilogue)
+                        wt[j] = ok ? (axis_f32(a.tau0, a.dtau, l) * a.wscale) : 0.f;
+                        sl1[j] = (!V3 && ok) ? a.slow[l] : 0.f;
                     }
 #pragma unroll
                     for (int b = 0; b < TB; ++b) {
@@ -153,46 +220,45 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                             const int l = lbase + j;
                             const bool ok = l < a.nt;
                             const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
-                            im[j] = ok ? a.image[idx] * (axis_f32(a.tau0, a.dtau, l) * a.wscale) : 0.f;
-                            sl[j] = ok ? (V3 ? a.slow[idx] : a.slow[l]) : 0.f;
+                            im[j] = ok ? ((a.image[idx] * wt[j]) * scale_a) * scale_b : 0.f;
+                            sl[j] = V3 ? (ok ? a.slow[idx] : 0.f) : sl1[j];
                         }
                         for (int g = 0; g < kAdjMax; ++g) {
                             const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
                             if (!((m >> b) & 1u)) continue;
                             const float2 ab = s_ab[g][b];
-                            Acc *acc = s_acc + g * a.nsa;
+                            unsigned long long *acc = s_acc + g * a.nsa;
                             // A lane's TL consecutive tau samples reach few
                             // distinct trace samples (at most one more per tau
                             // step where the time map is flat or slow): hold
                             // the running samples nc, nc+1 in registers and add
-                            // them to LDS when the index moves on (1.7x faster
-                            // than two atomics per update on a 2-D line).
+                            // them to LDS when the index moves on.
                             int nc = -2;
-                            Acc c0 = Acc(0), c1 = Acc(0);
+                            float c0 = 0.f, c1 = 0.f;
 #pragma unroll
                             for (int j = 0; j < TL; ++j) {
                                 const float ua = fmaf(ab.x, sl[j], tq[j]);
                                 const float ub = fmaf(ab.y, sl[j], tq[j]);
-                                float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
-                                x = fminf(x, a.nsf);
-                                int n = (int)x;
+                                const float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
+                                // a NaN time fails the test and is discarded
+                                const bool ok = x >= xlo && x < xhi;
+                                const int n = ok ? (int)x : a.ns;
                                 const float f = __builtin_amdgcn_fractf(x);
                                 const float w = __builtin_amdgcn_sqrtf(x) * im[j];
-                                n = (n >= lo && n <= hi) ? n : a.ns;   // a.ns, a.ns+1: discarded slots
-                                const float w1 = linear ? w * f : 0.f;
-                                const Acc v0 = Acc(w - w1), v1 = Acc(w1);
+                                const float w1 = LIN ? w * f : 0.f;
+                                const float v0 = w - w1, v1 = w1;
                                 if (n == nc) {
                                     c0 += v0;
                                     c1 += v1;
                                 } else if (n == nc + 1) {
-                                    atomicAdd(acc + adj_slot(nc), c0);
+                                    atomicAdd(acc + adj_slot(nc), to_fixed(c0));
                                     c0 = c1 + v0;
                                     c1 = v1;
                                     nc = n;
                                 } else {
                                     if (nc >= 0) {
-                                        atomicAdd(acc + adj_slot(nc), c0);
-                                        if (linear) atomicAdd(acc + adj_slot(nc + 1), c1);
+                                        atomicAdd(acc + adj_slot(nc), to_fixed(c0));
+                                        if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
                                     }
                                     c0 = v0;
                                     c1 = v1;
@@ -200,8 +266,8 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                 }
                             }
                             if (nc >= 0) {
-                                atomicAdd(acc + adj_slot(nc), c0);
-                                if (linear) atomicAdd(acc + adj_slot(nc + 1), c1);
+                                atomicAdd(acc + adj_slot(nc), to_fixed(c0));
+                                if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
                             }
                         }
                     }
@@ -210,10 +276,15 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
             __syncthreads();
         }
     }
+    const float inv_a = ldexpf(1.f, -Sa), inv_b = ldexpf(1.f, -Sb);
     for (int g = 0; g < ng; ++g) {
         float *out = a.traces + (size_t)s_k[g] * a.ns;
-        const Acc *acc = s_acc + g * a.nsa;
-        for (int n = tid; n < a.ns; n += NTH) out[n] = (float)((Acc)out[n] + acc[adj_slot(n)]);
+        const unsigned long long *acc = s_acc + g * a.nsa;
+        for (int n = tid; n < a.ns; n += NTH) {
+            const float v = bad ? __int_as_float(0x7fc00000)
+                                : (from_fixed(acc[adj_slot(n)]) * inv_a) * inv_b;
+            out[n] = out[n] + v;        // zeroed beforehand unless accumulating
+        }
     }
 }
 
@@ -452,9 +523,10 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     const int tb = p->tile_x * p->tile_y;
     const bool v3 = vel.kind == PSKTM_VEL_3D;
     const int nsa = adj_row(p->n_samples);
-    // FP64 accumulators in both modes: ds_add_f64 ran 2.7x faster than
-    // ds_add_f32 for this scatter on gfx950, and it is exact to FP32 inputs
-    const size_t lds = (size_t)p->adj_batch * nsa * sizeof(double);
+    // 64-bit fixed-point accumulators: on gfx950 ds_add_u64 runs this
+    // scatter's patterns at 2.3-2.8e12/s, ds_add_f64 at 1.2-2.7e12/s and
+    // ds_add_f32 at 2.0e11/s (bench.py --workload rates)
+    const size_t lds = (size_t)p->adj_batch * nsa * sizeof(unsigned long long);
     PSKTM_CHECK(lds <= 150 * 1024, PSKTM_E_UNSUPPORTED,
                 "traces of %d samples do not fit in LDS for the adjoint", p->n_samples);
     if ((s = prepare_tables(p, vel, st)) != PSKTM_OK) return s;
@@ -463,6 +535,14 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
         hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st,
                            traces_out, ntot);
     if (p->n_adj_items == 0) return check_launch("zero_kernel");
+    // the fixed-point scale: max |image * tau * sqrt(dt)|, stream-ordered
+    const long long nimg = (long long)g.nx * g.ny * g.nt;
+    const float wscale = (float)std::sqrt((double)p->dt);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, (float *)p->adj_wmax, (size_t)1);
+    hipLaunchKernelGGL(adj_max_kernel,
+                       dim3((unsigned)std::min<long long>((nimg + 255) / 256, 2048)), dim3(256),
+                       0, st, image, nimg, g.nt, g.tau0, g.dtau, wscale, p->adj_wmax);
+    if ((s = check_launch("adj_max_kernel")) != PSKTM_OK) return s;
 
     // 512 lanes x 4 consecutive tau samples: 16 waves per CU at two
     // workgroups (LDS-bound by the FP64 rows); 256 x 8 kept 8 waves per CU
@@ -477,9 +557,14 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
         if (atx >= aty) atx *= 2; else aty *= 2;
     }
     const int atb = atx * aty;
-    const AdjFn fn = (atb == 8)
-        ? (v3 ? demigrate_kernel<NTH, 8, TL, true, double> : demigrate_kernel<NTH, 8, TL, false, double>)
-        : (v3 ? demigrate_kernel<NTH, 16, TL, true, double> : demigrate_kernel<NTH, 16, TL, false, double>);
+    const bool lin = p->opt.interp == PSKTM_INTERP_LINEAR;
+    AdjFn fn;
+    if (atb == 8)
+        fn = v3 ? (lin ? demigrate_kernel<NTH, 8, TL, true, true> : demigrate_kernel<NTH, 8, TL, true, false>)
+                : (lin ? demigrate_kernel<NTH, 8, TL, false, true> : demigrate_kernel<NTH, 8, TL, false, false>);
+    else
+        fn = v3 ? (lin ? demigrate_kernel<NTH, 16, TL, true, true> : demigrate_kernel<NTH, 16, TL, true, false>)
+                : (lin ? demigrate_kernel<NTH, 16, TL, false, true> : demigrate_kernel<NTH, 16, TL, false, false>);
     AdjArgs a{};
     a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
     a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;
@@ -490,7 +575,11 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     a.win_x = p->win_x; a.win_y = p->win_y;
     a.nsf = (float)p->n_samples;
     a.r2 = p->opt.aperture * p->opt.aperture;
-    a.wscale = (float)std::sqrt((double)p->dt);
+    a.wscale = wscale;
+    // |contribution| <= sqrt(x) * max|image * w| with x < ns, and a slot sums
+    // at most every update of its trace (<= nx * ny * nt)
+    a.bound = (float)(1.01 * std::sqrt((double)p->n_samples) * (double)g.nx * g.ny * g.nt);
+    a.wmax = p->adj_wmax;
     a.use_ap = !p->all_traces;
     a.interp = p->opt.interp;
     a.nsa = nsa;

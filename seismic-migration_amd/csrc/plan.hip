@@ -245,7 +245,8 @@ static void plan_free(psktm_plan *p)
 {
     if (!p) return;
     void *bufs[] = {p->geo, p->sorted_ids, p->cell_start, p->tile_off, p->list,
-                    p->tile_order, p->adj_items, p->slow, p->tsq, p->partial, p->reach};
+                    p->tile_order, p->adj_items, p->slow, p->tsq, p->partial, p->reach,
+                    p->adj_wmax};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -305,6 +306,7 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
     PLAN_HIP(dalloc(p, &p->sorted_ids, (size_t)ntr));
     PLAN_HIP(dalloc(p, &p->cell_start, (size_t)p->n_cells + 1));
     PLAN_HIP(dalloc(p, &p->reach, 1));
+    PLAN_HIP(dalloc(p, &p->adj_wmax, 1));
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, st, (int *)p->reach, 1ll, 0);
     const int nb = 256;
     const int gtr = (ntr + nb - 1) / nb;

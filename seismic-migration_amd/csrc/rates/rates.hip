@@ -13,7 +13,7 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsSlots = 4096;      // 16 KiB of floats / 32 KiB of doubles
+constexpr int kLdsSlots = 4096;      // 16 KiB of 32-bit / 32 KiB of 64-bit slots
 
 // 8 independent FMA chains per lane: 8 * iters FMAs.
 __global__ __launch_bounds__(kThreads) void fma_kernel(int iters, float *out)
@@ -203,7 +203,8 @@ extern "C" {
 //     (8*iters sqrt per lane)
 //   4 LDS ds_read_b64 random (8*iters per lane)
 //   5 v_fma_f32, 6 v_add_f32, 7 v_mul_f32, every operand a VGPR (8*iters per lane)
-//   10+p LDS ds_add_f32, 20+p ds_add_f64, 30+p ds_add_u32 with pattern p
+//   10+p LDS ds_add_f32, 20+p ds_add_f64, 30+p ds_add_u32, 40+p ds_add_u64
+//     with pattern p
 //     (8*iters per lane)
 // Returns 0 on success.
 int psktm_rates_issue(int op, int blocks, int iters, int lds_bytes, float *ms)
@@ -247,13 +248,15 @@ int psktm_rates_run(int kind, int blocks, int iters, float *ms)
     case 7: rc = timed([&] { hipLaunchKernelGGL(valu_kernel<2>, g, t, 0, 0, iters, out, 1.0f); }, ms); break;
     default: {
         const int p = kind % 10, ty = kind / 10;
-        if (p > 2 || ty < 1 || ty > 3) { rc = 11; break; }
+        if (p > 2 || ty < 1 || ty > 4) { rc = 11; break; }
         if (ty == 1)
             rc = timed([&] { hipLaunchKernelGGL(lds_atomic_kernel<float>, g, t, 0, 0, iters, p, out); }, ms);
         else if (ty == 2)
             rc = timed([&] { hipLaunchKernelGGL(lds_atomic_kernel<double>, g, t, 0, 0, iters, p, out); }, ms);
-        else
+        else if (ty == 3)
             rc = timed([&] { hipLaunchKernelGGL(lds_atomic_kernel<unsigned>, g, t, 0, 0, iters, p, out); }, ms);
+        else
+            rc = timed([&] { hipLaunchKernelGGL(lds_atomic_kernel<unsigned long long>, g, t, 0, 0, iters, p, out); }, ms);
     }
     }
     (void)hipFree(out);

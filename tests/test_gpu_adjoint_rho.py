@@ -22,9 +22,9 @@ def dot64(a, b):
 def test_dot_product_test_3d(cuda, interp, precise, tol):
     """<F d, m> = <d, F' m> with zero-mean random d, m and FP64 inner products.
 
-    The adjoint accumulates in FP64 in both modes. Precise mode (Kahan
-    forward, split-K partials reduced with their low-order parts in FP64)
-    meets 1e-6. The plain forward's FP32 sum of K ~ 2000 random-sign terms per
+    The adjoint computes in FP32 and adds its contributions exactly (64-bit
+    fixed point) in both modes. Precise mode (Kahan forward, split-K partials
+    reduced with their low-order parts in FP64) meets 1e-6. The plain forward's FP32 sum of K ~ 2000 random-sign terms per
     image sample has a relative error ~ eps*sqrt(K/2) ~ 2e-6 per sample, and
     |<d,F'm>| here is a small fraction of the sum of its absolute terms, so the
     plain-mode figure depends on the summation order, i.e. on the tile shape:
@@ -106,6 +106,43 @@ def test_adjoint_without_aperture_vs_fp64_oracle(cuda, tiles):
                               vel, dt, tr.shape[1], vel3d=True, aperture=0.0,
                               interp=oracle.LINEAR)
     assert rel_l2(got, ref) <= 1e-4
+
+
+@pytest.mark.parametrize("interp", [psktm.INTERP_LINEAR, psktm.INTERP_TRUNC])
+def test_adjoint_bits_accumulate_and_extreme_images(cuda, interp):
+    """The adjoint adds exact fixed-point integers, so its output does not
+    depend on the order of the LDS atomics: repeated calls give the same bits.
+    Accumulate mode adds onto the existing traces (bit-identical to adding
+    the overwritten result in FP32). Images scaled by 2^100 and 2^-100 give
+    the same traces scaled by the same power of two (the fixed-point scale
+    follows the image); a NaN or inf in the image makes the covered traces
+    NaN; a zero image gives zero traces."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=17, ntr=700, nx=8, ny=7, nt=200, ns=240)
+    plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, interp=interp)
+    g = torch.Generator(device="cpu").manual_seed(17)
+    m = torch.randn(grid[0], grid[3], grid[6], generator=g).to(cuda)
+    v = t(vel)
+    y = psktm.demigrate(plan, m, v)
+    for _ in range(3):
+        assert torch.equal(psktm.demigrate(plan, m, v).view(torch.int32), y.view(torch.int32))
+    assert y.abs().max() > 0
+    y0 = torch.randn(y.shape, generator=g).to(cuda)
+    acc = y0.clone()
+    psktm.demigrate(plan, m, v, acc, accumulate=True)
+    assert torch.equal(acc.view(torch.int32), (y0 + y).view(torch.int32))
+    for p2 in (100, -100):
+        ys = psktm.demigrate(plan, m * 2.0 ** p2, v).double() * 2.0 ** -p2
+        e = float((ys - y.double()).norm() / y.double().norm())
+        print(f"image x 2^{p2}: rel-L2 of the rescaled traces {e:.2e}")
+        assert e <= 1e-6, (p2, e)
+    covered = y.abs().amax(dim=1) > 0
+    for bad in (float("nan"), float("inf")):
+        mb = m.clone()
+        mb[3, 2, 100] = bad
+        yb = psktm.demigrate(plan, mb, v)
+        assert torch.isnan(yb[covered]).all()
+    assert torch.count_nonzero(psktm.demigrate(plan, torch.zeros_like(m), v)) == 0
 
 
 @pytest.mark.parametrize("seed", [3, 4])
