@@ -170,8 +170,8 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const int tj0 = max(0, CJ - a.cell_my - a.win_y), tj1 = min(a.tiles_y - 1, CJ - a.cell_my + a.win_y);
     // valid sample index: LINEAR 0 <= n <= ns-2, TRUNC 1 <= n <= ns-1
     // (src/psktmCPU.c:82); anything else goes to the discarded slots ns, ns+1
-    const float xhi = LIN ? a.nsf - 1.f : a.nsf;
-    const float xlo = LIN ? 0.f : 1.f;
+    const float xhi = a.nsf - 1.f;
+    const float xmid = 0.5f * (a.nsf + 1.f), xrad = 0.5f * (a.nsf - 1.f);
 
     for (int TI = ti0; TI <= ti1; ++TI) {
         for (int TJ = tj0; TJ <= tj1; ++TJ) {
@@ -233,42 +233,50 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                             // step where the time map is flat or slow): hold
                             // the running samples nc, nc+1 in registers and add
                             // them to LDS when the index moves on.
-                            int nc = -2;
+                            int nc = 0;
                             float c0 = 0.f, c1 = 0.f;
 #pragma unroll
                             for (int j = 0; j < TL; ++j) {
                                 const float ua = fmaf(ab.x, sl[j], tq[j]);
                                 const float ub = fmaf(ab.y, sl[j], tq[j]);
                                 const float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
-                                // a NaN time fails the test and is discarded
-                                const bool ok = x >= xlo && x < xhi;
+                                // LINEAR: 0 <= x < ns-1 (x >= 0 holds); TRUNC:
+                                // 1 <= x < ns, as |x - (ns+1)/2| < (ns-1)/2. A
+                                // NaN time fails both and is discarded.
+                                const bool ok = LIN ? x < xhi : fabsf(x - xmid) < xrad;
                                 const int n = ok ? (int)x : a.ns;
                                 const float f = __builtin_amdgcn_fractf(x);
                                 const float w = __builtin_amdgcn_sqrtf(x) * im[j];
                                 const float w1 = LIN ? w * f : 0.f;
                                 const float v0 = w - w1, v1 = w1;
-                                if (n == nc) {
-                                    c0 += v0;
-                                    c1 += v1;
-                                } else if (n == nc + 1) {
-                                    atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                    c0 = c1 + v0;
-                                    c1 = v1;
+                                if (j == 0) {
                                     nc = n;
-                                } else {
-                                    if (nc >= 0) {
-                                        atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                        if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
-                                    }
                                     c0 = v0;
                                     c1 = v1;
-                                    nc = n;
+                                    continue;
                                 }
+                                const bool same = n == nc, next = n == nc + 1;
+                                // the rare jump (more than one sample, or back):
+                                // flush both running samples, start afresh
+                                const bool jump = !(same || next);
+                                if (__builtin_amdgcn_ballot_w64(jump)) {
+                                    if (jump) {
+                                        atomicAdd(acc + adj_slot(nc), to_fixed(c0));
+                                        if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
+                                        c0 = 0.f;
+                                        c1 = 0.f;
+                                    }
+                                }
+                                // one sample on: sample nc is complete
+                                if (__builtin_amdgcn_ballot_w64(next)) {
+                                    if (next) atomicAdd(acc + adj_slot(nc), to_fixed(c0));
+                                }
+                                c0 = (same ? c0 : (next ? c1 : 0.f)) + v0;
+                                c1 = (same ? c1 : 0.f) + v1;
+                                nc = n;
                             }
-                            if (nc >= 0) {
-                                atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
-                            }
+                            atomicAdd(acc + adj_slot(nc), to_fixed(c0));
+                            if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
                         }
                     }
                 }
