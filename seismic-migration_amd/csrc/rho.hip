@@ -7,8 +7,11 @@
 // :50, :57), so with H_nyq replaced by Re H_nyq the product Y = H Z inverts to
 // filtered a in the real part and filtered b in the imaginary part: one
 // forward and one inverse FFT serve two traces, with no spectrum separation.
-// The FFT is a radix-2 Stockham transform ping-ponging between two LDS
-// buffers, twiddles from a table computed on the host in double precision.
+// For M = 1024, 2048 and 4096 (traces of 513-4096 samples) the FFT is a
+// register-blocked Stockham transform of radix 16, 16 and M/256 (described
+// at rho_fast_kernel); other M use a radix-2 Stockham transform ping-ponging
+// between two LDS buffers. Twiddles come from a table computed on the host
+// in double precision.
 //
 // Frequencies follow the reference exactly: w_k = 2 pi k' / Dr with
 // k' = k (k < M/2), k - M (k > M/2), -M/2 at Nyquist, Dr = dw * 1e-6 (:31-33),
@@ -23,7 +26,9 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <mutex>
 #include <vector>
 
@@ -32,10 +37,18 @@ namespace {
 
 constexpr int kRhoThreads = 256;
 constexpr int kRhoMaxM = 4096;
+// resident waves per SIMD asked of the radix-16 kernel (at most 256 VGPRs):
+// two 256-lane workgroups per CU, which their LDS (43-68 KB each) allows
+constexpr int kRhoWaves = 2;
 
+// Complex product with its two fused multiply-adds written out: the radix-16
+// kernel is compiled without contraction so that every output type computes
+// the same bits (left to the compiler, contraction differs between the
+// instantiations).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+#pragma clang fp contract(off)
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 
 // In-LDS radix-2 Stockham FFT of length M (power of two); sign -1 forward,
@@ -82,10 +95,18 @@ __device__ __forceinline__ float block_max(float v, float *s_red)
     return r;
 }
 
+// y * inv rounded to binary32, then to the output type (nearest even). Left
+// alone, the compiler fuses the product and the FP16 conversion into one
+// mixed-precision FMA (v_fma_mixlo_f16) that rounds once and so can differ on
+// ties; an empty asm statement holds the binary32 product in a register.
 __device__ __forceinline__ void store_q(void *out, int dtype, size_t idx, float y, float inv)
 {
     if (dtype == PSKTM_F32) static_cast<float *>(out)[idx] = y;
-    else if (dtype == PSKTM_F16) static_cast<__half *>(out)[idx] = __float2half_rn(y * inv);
+    else if (dtype == PSKTM_F16) {
+        float p = y * inv;
+        asm volatile("" : "+v"(p));
+        static_cast<__half *>(out)[idx] = __float2half_rn(p);
+    }
     else static_cast<unsigned short *>(out)[idx] = f2bf16_rne(y * inv);
 }
 
@@ -97,6 +118,26 @@ __device__ __forceinline__ float row_gain(int n, float gain_dt)
     if (gain_dt == 0.0f) return 1.0f;
     const float t = (float)n * gain_dt;
     return sqrtf(t);
+}
+
+// The filter at bin k of an M-point transform (scripts/ktmApplyRhoFilter.m:
+// 31-58): w = 2 pi k' / Dr with k' = k (k < M/2), k - M (k >= M/2, so the
+// Nyquist bin is -M/2); opt 1 |w|, opt 2 sqrt|w| e^{i pi/4 sgn w}, opt 3 i w;
+// DC -> 0; at the Nyquist bin the real() keeps Re H only.
+__device__ __forceinline__ float2 rho_response(int k, int M, float wscale, int opt)
+{
+    if (k == 0) return make_float2(0.f, 0.f);
+    const int kp = (k < (M >> 1)) ? k : k - M;
+    const float w = wscale * (float)kp;
+    const float aw = fabsf(w);
+    float2 h;
+    if (opt == 1) h = make_float2(aw, 0.f);
+    else if (opt == 2) {
+        const float r = sqrtf(aw) * 0.70710678118654752f;
+        h = make_float2(r, w < 0.f ? -r : r);
+    } else h = make_float2(0.f, w);
+    if (k == (M >> 1)) h.y = 0.f;
+    return h;
 }
 
 __global__ __launch_bounds__(kRhoThreads) void rho_kernel(
@@ -126,26 +167,8 @@ __global__ __launch_bounds__(kRhoThreads) void rho_kernel(
     float2 *Z = stockham(b0, b1, tw, M, logM, false);
     float2 *other = (Z == b0) ? b1 : b0;
     // Y_k = H_k Z_k (natural order), written into the other buffer's partner
-    const float c45 = 0.70710678118654752f;
-    for (int k = threadIdx.x; k < M; k += blockDim.x) {
-        float2 z = Z[k], y;
-        if (k == 0) {
-            y = make_float2(0.f, 0.f);
-        } else {
-            const int kp = (k < half) ? k : k - M;        // Nyquist -> -M/2
-            const float w = wscale * (float)kp;           // 2 pi k' / Dr
-            const float aw = fabsf(w);
-            float2 h;
-            if (opt == 1) h = make_float2(aw, 0.f);
-            else if (opt == 2) {
-                const float r = sqrtf(aw) * c45;
-                h = make_float2(r, w < 0.f ? -r : r);
-            } else h = make_float2(0.f, w);
-            if (k == half) h.y = 0.f;                     // real() keeps Re H only
-            y = cmul(h, z);
-        }
-        other[k] = y;
-    }
+    for (int k = threadIdx.x; k < M; k += blockDim.x)
+        other[k] = k == 0 ? make_float2(0.f, 0.f) : cmul(rho_response(k, M, wscale, opt), Z[k]);
     __syncthreads();
     float2 *buf_a = other, *buf_b = Z;
     float2 *Y = stockham(buf_a, buf_b, tw, M, logM, true);
@@ -218,6 +241,256 @@ __global__ void ibm_kernel(const unsigned *__restrict__ in, float *__restrict__ 
     out[i] = (w & 0x80000000u) ? -v : v;
 }
 
+// ---- register-blocked Stockham FFT for M = 1024, 2048, 4096 -----------------
+//
+// M = 16 x 16 x R3 (R3 = M/256): three Stockham passes of radix 16, 16 and R3
+// (pass p reads x[jj + r M/R], twiddles by W_M^(r k M/(Ns R)) with k = jj mod
+// Ns, and writes the R outputs to (jj/Ns) Ns R + k + s Ns; Ns = 1, 16, 256).
+// L = M/16 lanes serve one complex FFT (two real traces): one butterfly per
+// lane in passes 1-2, 16/R3 in pass 3, so that after the forward transform
+// lane j holds X[j + L r] for r = 0..15 in registers. That is exactly the
+// input of the first inverse pass, so the filter is applied in registers and
+// a forward-inverse pair needs four LDS exchanges (the radix-2 form: 22
+// passes, each through LDS behind a barrier). The inverse leaves lane j with
+// y[j + L r], stored with consecutive lanes on consecutive samples.
+
+// W_16^e = exp(-2 pi i e / 16) (forward) for e = 0..7; INV conjugates.
+template <int E, bool INV>
+__device__ __forceinline__ float2 rot16(float2 v)
+{
+#pragma clang fp contract(off)
+    constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f;
+    constexpr float c2 = 0.70710678118654752f;
+    constexpr float sg = INV ? -1.f : 1.f;      // forward multiplies by cos - i sin
+    if constexpr (E == 0) return v;
+    else if constexpr (E == 4) return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+    else if constexpr (E == 2 || E == 6) {
+        // (cos, -sg sin) with cos = +-c2, sin = c2
+        const float c = E == 2 ? c2 : -c2;
+        const float sn = sg * c2;
+        return make_float2(fmaf(v.x, c, v.y * sn), fmaf(v.y, c, -(v.x * sn)));
+    } else {
+        const float c = (E == 1) ? c1 : (E == 3) ? s1 : (E == 5) ? -s1 : -c1;
+        const float sn = sg * ((E == 1 || E == 7) ? s1 : c1);
+        return make_float2(fmaf(v.x, c, v.y * sn), fmaf(v.y, c, -(v.x * sn)));
+    }
+}
+
+// In-register DFT of R = 4, 8 or 16 points (radix-2 decimation in
+// frequency, outputs put back in natural order at compile time).
+template <int R, bool INV>
+__device__ __forceinline__ void dft_regs(float2 (&v)[16])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int span = R / 2; span >= 1; span >>= 1) {
+#pragma unroll
+        for (int st = 0; st < R; st += 2 * span) {
+#pragma unroll
+            for (int i = 0; i < span; ++i) {
+                const float2 a = v[st + i], b = v[st + i + span];
+                v[st + i] = make_float2(a.x + b.x, a.y + b.y);
+                const float2 d = make_float2(a.x - b.x, a.y - b.y);
+                // W_{2 span}^i = W_16^(i * 8 / span)
+                const int e = i * (8 / span);
+                float2 r;
+                switch (e) {
+                case 0: r = rot16<0, INV>(d); break;
+                case 1: r = rot16<1, INV>(d); break;
+                case 2: r = rot16<2, INV>(d); break;
+                case 3: r = rot16<3, INV>(d); break;
+                case 4: r = rot16<4, INV>(d); break;
+                case 5: r = rot16<5, INV>(d); break;
+                case 6: r = rot16<6, INV>(d); break;
+                default: r = rot16<7, INV>(d); break;
+                }
+                v[st + i + span] = r;
+            }
+        }
+    }
+    // bit-reversed -> natural order
+    float2 t[16];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int b = 0;
+#pragma unroll
+        for (int q = 1, r = R >> 1; q < R; q <<= 1, r >>= 1)
+            if (k & q) b |= r;
+        t[k] = v[b];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = t[k];
+}
+
+// LDS index with one pad slot per 16 entries (the 16-apart writes of the
+// radix-16 passes would otherwise fall on one bank).
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+
+// One Stockham pass on L lanes (lane j): radix R, span Ns, Q = 16/R
+// butterflies per lane (jj = j + L q). Inputs come from regs (FROM_REGS:
+// input r is regs[r]) or from LDS; outputs go to LDS, or (TO_REGS) output s
+// of butterfly q to regs[q + Q s]. In place (src == dst) every lane reads all
+// its inputs, then the workgroup meets at a barrier, then writes.
+template <int M, int R, int Ns, bool INV, bool FROM_REGS, bool TO_REGS>
+__device__ __forceinline__ void stockham_pass(float2 (&regs)[16], const float2 *src,
+                                              float2 *dst, const float2 *__restrict__ tw, int j)
+{
+#pragma clang fp contract(off)
+    constexpr int L = M / 16, Q = 16 / R, STRIDE = M / R;
+    static_assert((M / R) == L * Q, "one butterfly per lane and input slot");
+    float2 vin[16];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            vin[q * R + r] = FROM_REGS ? regs[r] : src[pidx(j + L * q + r * STRIDE)];
+    }
+    if (!FROM_REGS && !TO_REGS) __syncthreads();          // in place: reads before writes
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int jj = j + L * q;
+        const int k = jj & (Ns - 1);
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = vin[q * R + r];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[(r * k * (M / (Ns * R))) & (M - 1)];
+                if (INV) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_regs<R, INV>(v);
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            if (TO_REGS) regs[q + Q * s] = v[s];
+            else dst[pidx((jj / Ns) * Ns * R + k + s * Ns)] = v[s];
+        }
+    }
+}
+
+// One workgroup of 256 lanes runs 256/L transforms at a time (two traces
+// each) and loops over trace groups; the twiddle table W_M^e (e < M) and one
+// padded exchange buffer per transform live in LDS.
+template <int M, int DT>
+__global__ __launch_bounds__(kRhoThreads) __attribute__((amdgpu_waves_per_eu(kRhoWaves)))
+void rho_fast_kernel(
+    const float *__restrict__ in, int n, int m, float wscale, int opt, float qmax,
+    void *__restrict__ out, float *__restrict__ scale_out, int shift,
+    const float2 *__restrict__ twg, float gain_dt)
+{
+#pragma clang fp contract(off)
+    constexpr int L = M / 16, FPW = kRhoThreads / L, R3 = M / 256;
+    constexpr int BUF = M + M / 16;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    __shared__ float s_red[2][kRhoThreads / 64];
+    const int tid = threadIdx.x, slot = tid / L, j = tid % L;
+    float2 *buf = smem + M + slot * BUF;
+    for (int q = tid; q < M; q += kRhoThreads) tw[q] = twg[q];
+    const int rot = (shift && (n & 1)) ? 1 : 0;
+    const float invM = 1.0f / (float)M;
+    const int pairs = (n + 1) / 2;
+    for (int p0 = blockIdx.x * FPW; p0 < pairs; p0 += gridDim.x * FPW) {
+        const int pr = p0 + slot;
+        const bool live = pr < pairs;
+        const int ra = 2 * pr, rb = ra + 1;
+        const bool has_b = live && rb < n;
+        // a transform spans whole waves (L >= 64): its rows are wave-uniform,
+        // so the row bases live in scalar registers and each load needs only
+        // a 32-bit lane offset
+        const int ia = __builtin_amdgcn_readfirstlane(live ? (ra + rot) % n : 0);
+        const int ib = __builtin_amdgcn_readfirstlane(has_b ? (rb + rot) % n : 0);
+        const float *rowa = in + (size_t)ia * m, *rowb = in + (size_t)ib * m;
+        float2 regs[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int q = j + L * r;
+            float va = 0.f, vb = 0.f;
+            if (live && q < m) {
+                va = rowa[q];
+                if (has_b) vb = rowb[q];
+            }
+            regs[r] = make_float2(va, vb);
+        }
+        __syncthreads();                       // previous group done with buf (and tw loaded)
+        stockham_pass<M, 16, 1, false, true, false>(regs, nullptr, buf, tw, j);
+        __syncthreads();
+        stockham_pass<M, 16, 16, false, false, false>(regs, buf, buf, tw, j);
+        __syncthreads();
+        stockham_pass<M, R3, 256, false, false, true>(regs, buf, nullptr, tw, j);
+        // filter in registers: lane j holds X[j + L r]
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int k = j + L * r;
+            regs[r] = k == 0 ? make_float2(0.f, 0.f) : cmul(rho_response(k, M, wscale, opt), regs[r]);
+        }
+        __syncthreads();                       // pass-3 reads done before buf is rewritten
+        stockham_pass<M, 16, 1, true, true, false>(regs, nullptr, buf, tw, j);
+        __syncthreads();
+        stockham_pass<M, 16, 16, true, false, false>(regs, buf, buf, tw, j);
+        __syncthreads();
+        stockham_pass<M, R3, 256, true, false, true>(regs, buf, nullptr, tw, j);
+        // lane j holds y[j + L r] (times M): gain, per-trace max-abs, then the
+        // values go through the exchange buffer in natural order, so that
+        // the stores are a plain loop with consecutive lanes on consecutive
+        // samples
+        float ma = 0.f, mb = 0.f;
+        __syncthreads();                       // pass-3 reads done before buf is rewritten
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int q = j + L * r;
+            const float gq = row_gain(q, gain_dt);
+            const float ya = (regs[r].x * invM) * gq, yb = (regs[r].y * invM) * gq;
+            buf[pidx(q)] = make_float2(ya, yb);
+            if (q < m) {
+                ma = fmaxf(ma, fabsf(ya));
+                mb = fmaxf(mb, fabsf(yb));
+            }
+        }
+        float inv_a = 1.f, inv_b = 1.f;
+        if (DT != PSKTM_F32) {
+            // max over the L lanes of this transform: waves, then LDS
+            for (int o = 32; o > 0; o >>= 1) {
+                ma = fmaxf(ma, __shfl_xor(ma, o));
+                mb = fmaxf(mb, __shfl_xor(mb, o));
+            }
+            if (L > 64) {
+                if ((tid & 63) == 0) {
+                    s_red[0][tid >> 6] = ma;
+                    s_red[1][tid >> 6] = mb;
+                }
+                __syncthreads();
+                ma = 0.f;
+                mb = 0.f;
+                const int w0 = slot * (L / 64);
+                for (int w = 0; w < L / 64; ++w) {
+                    ma = fmaxf(ma, s_red[0][w0 + w]);
+                    mb = fmaxf(mb, s_red[1][w0 + w]);
+                }
+            }
+            const float sa = ma / qmax, sb = mb / qmax;
+            inv_a = (sa > 0.f) ? 1.0f / sa : 0.f;
+            inv_b = (sb > 0.f) ? 1.0f / sb : 0.f;
+            if (live && j == 0) {
+                scale_out[ra] = sa;
+                if (has_b) scale_out[rb] = sb;
+            }
+        }
+        __syncthreads();
+        if (live) {
+            const size_t oa = (size_t)__builtin_amdgcn_readfirstlane(ra) * m;
+            const size_t ob = oa + m;
+            for (int q = j; q < m; q += L) {
+                const float2 y = buf[pidx(q)];
+                store_q(out, DT, oa + q, y.x, inv_a);
+                if (has_b) store_q(out, DT, ob + q, y.y, inv_b);
+            }
+        }
+    }
+}
+
 // Twiddle tables per (device, log2 M); kept for the life of the process so
 // that a kernel still in flight never reads a freed table.
 constexpr int kMaxDevices = 64;
@@ -231,14 +504,16 @@ psktm_status twiddles(int dev, int logM, const float2 **out)
     float2 *&slot = g_tw[dev][logM];
     if (!slot) {
         const int M = 1 << logM;
-        std::vector<float2> h(M / 2);
-        for (int q = 0; q < M / 2; ++q) {
+        // W_M^q = exp(-2 pi i q / M) for q < M (the radix-2 passes read the
+        // first half, the radix-16 passes all of it)
+        std::vector<float2> h(M);
+        for (int q = 0; q < M; ++q) {
             const double a = -2.0 * M_PI * q / M;
             h[q] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
         float2 *d = nullptr;
-        PSKTM_HIP(hipMalloc(&d, sizeof(float2) * (M / 2)));
-        hipError_t e = hipMemcpy(d, h.data(), sizeof(float2) * (M / 2), hipMemcpyHostToDevice);
+        PSKTM_HIP(hipMalloc(&d, sizeof(float2) * M));
+        hipError_t e = hipMemcpy(d, h.data(), sizeof(float2) * M, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             (void)hipFree(d);
             return hip_fail(e, "hipMemcpy(twiddles)", __FILE__, __LINE__);
@@ -297,6 +572,34 @@ psktm_status psktm_rho_filter_gain(const float *in, int n_traces, int n_samples,
     if ((st = twiddles(dev, logM, &tw)) != PSKTM_OK) return st;
     const double Dr = (double)dw_us * 1e-6;
     const float wscale = (float)(2.0 * M_PI / Dr);
+    if (M == 1024 || M == 2048 || M == 4096) {
+        // register-blocked radix-16 form; a grid of a few workgroups per CU
+        // loops over the trace pairs (one twiddle-table load per workgroup)
+        auto pick = [&](auto mtag) -> const void * {
+            constexpr int MM = decltype(mtag)::value;
+            return out_dtype == PSKTM_F32 ? (const void *)rho_fast_kernel<MM, PSKTM_F32>
+                 : out_dtype == PSKTM_F16 ? (const void *)rho_fast_kernel<MM, PSKTM_F16>
+                                          : (const void *)rho_fast_kernel<MM, PSKTM_BF16>;
+        };
+        const void *fn = M == 1024 ? pick(std::integral_constant<int, 1024>{})
+                       : M == 2048 ? pick(std::integral_constant<int, 2048>{})
+                                   : pick(std::integral_constant<int, 4096>{});
+        const int L = M / 16, fpw = kRhoThreads / L;
+        const size_t lds = sizeof(float2) * ((size_t)M + (size_t)fpw * (M + M / 16));
+        { psktm_status s_; if ((s_ = ensure_dynamic_lds(fn, lds)) != PSKTM_OK) return s_; }
+        int cus = 0;
+        PSKTM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long long groups = ((long long)(n_traces + 1) / 2 + fpw - 1) / fpw;
+        const unsigned blocks = (unsigned)std::max(1ll, std::min(groups, 8ll * cus));
+        const float qmax = qmax_of(out_dtype);
+        const int shift = matlab_trace_shift ? 1 : 0;
+        void *args[] = {(void *)&in, (void *)&n_traces, (void *)&n_samples, (void *)&wscale,
+                        (void *)&opt, (void *)&qmax, (void *)&out,
+                        (void *)&scale_out, (void *)&shift, (void *)&tw, (void *)&gain_dt};
+        PSKTM_HIP(hipLaunchKernel(fn, dim3(blocks), dim3(kRhoThreads), args, lds,
+                                  as_stream(stream)));
+        return check_launch("rho_fast_kernel");
+    }
     const size_t lds = sizeof(float2) * (2 * (size_t)M + M / 2);
     { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)rho_kernel, lds)) != PSKTM_OK) return s_; }
     const unsigned blocks = (unsigned)((n_traces + 1) / 2);

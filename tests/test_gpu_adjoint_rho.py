@@ -172,7 +172,8 @@ def test_dot_product_line_2d_no_aperture(cuda, seed):
 # ----------------------------- rho filter --------------------------------
 
 @pytest.mark.parametrize("opt", [1, 2, 3])
-@pytest.mark.parametrize("n,m", [(6, 1000), (5, 1500), (1, 37)])
+@pytest.mark.parametrize("n,m", [(6, 1000), (5, 1500), (1, 37), (3, 3000), (9, 2048),
+                                 (2, 4096), (17, 700)])
 def test_rho_filter_vs_oracle(cuda, opt, n, m):
     import torch
     rng = np.random.default_rng(opt * 100 + n)

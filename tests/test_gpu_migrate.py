@@ -298,13 +298,17 @@ def _gain_np(x, dt):
     return (x.astype(np.float32) * g[None]).astype(np.float32)
 
 
-def test_trace_gain_and_rho_gain_bit_exact(cuda):
+@pytest.mark.parametrize("m", [333, 1500, 4000])
+def test_trace_gain_and_rho_gain_bit_exact(cuda, m):
     """psktm_trace_gain is in[k][n] * sqrtf(n dt) in binary32; the rho filter's
     fused gain equals the ungained filter output times the same factors, and its
-    FP16 form equals quantising the FP32 gained output."""
+    FP16 form equals quantising the FP32 gained output (radix-2 transform at
+    m = 333, radix-16 at 1500 and 4000). 300 traces hold a few dozen values
+    whose binary32 product y * (1/s) is an FP16 tie: rounding the product
+    once from its exact value (a fused mixed-precision FMA) fails on them."""
     import torch
     rng = np.random.default_rng(41)
-    x = rng.standard_normal((7, 333)).astype(np.float32)
+    x = rng.standard_normal((300, m)).astype(np.float32)
     dt = 0.004
     xt = torch.as_tensor(x).to(cuda)
     got = psktm.trace_gain(xt, dt).cpu().numpy()
