@@ -84,9 +84,10 @@ __device__ __forceinline__ float axis_f32(float start, float step, int i)
 } // namespace psktm
 
 // A migration plan: geometry binned into output tiles (see psktm_plan_create).
-// 64-bit slots per adjoint accumulator row for traces of ns samples: slots for
-// samples 0 .. ns+1 (the last two take discarded contributions) with one pad
-// slot every 16 samples.
+// 64-bit slots per adjoint accumulator row for traces of ns samples: two
+// slots that take discarded contributions, then samples 0 .. ns-1 with one
+// pad slot every 16 samples (an upper bound; the kernel's layout is
+// kAdjFront + adj_slot(n)).
 inline int adj_row(int ns) { return (ns + 2) + ((ns + 2) >> 4) + 1; }
 
 struct psktm_plan {

@@ -53,6 +53,9 @@ constexpr int kAdjMax = 8;    // traces per adjoint work item (upper bound)
 // every 16 samples, so that the regular sample strides of a wave's scatter do
 // not fall on the same banks (5% faster on a 2-D line than a plain row).
 __host__ __device__ inline int adj_slot(int n) { return n + (n >> 4); }
+// Row entries ahead of sample 0: adj_slot(-2) = -3 and adj_slot(-1) = -2 are
+// the discarded slots. kAdjFront + adj_slot(ns - 1) + 1 <= adj_row(ns).
+constexpr int kAdjFront = 3;
 
 struct AdjArgs {
     int nx, ny, nt, ns;
@@ -169,7 +172,9 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const int ti0 = max(0, CI - a.cell_mx - a.win_x), ti1 = min(a.tiles_x - 1, CI - a.cell_mx + a.win_x);
     const int tj0 = max(0, CJ - a.cell_my - a.win_y), tj1 = min(a.tiles_y - 1, CJ - a.cell_my + a.win_y);
     // valid sample index: LINEAR 0 <= n <= ns-2, TRUNC 1 <= n <= ns-1
-    // (src/psktmCPU.c:82); anything else goes to the discarded slots ns, ns+1
+    // (src/psktmCPU.c:82); anything else goes to sample -2: the discarded
+    // slots of samples -2 and -1 sit ahead of sample 0 in each row (row
+    // entry adj_slot(n) + kAdjFront; an inline constant, unlike ns)
     const float xhi = a.nsf - 1.f;
     const float xmid = 0.5f * (a.nsf + 1.f), xrad = 0.5f * (a.nsf - 1.f);
 
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                             const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
                             if (!((m >> b) & 1u)) continue;
                             const float2 ab = s_ab[g][b];
-                            unsigned long long *acc = s_acc + g * a.nsa;
+                            unsigned long long *acc = s_acc + g * a.nsa + kAdjFront;
                             // A lane's TL consecutive tau samples reach few
                             // distinct trace samples (at most one more per tau
                             // step where the time map is flat or slow): hold
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                 // 1 <= x < ns, as |x - (ns+1)/2| < (ns-1)/2. A
                                 // NaN time fails both and is discarded.
                                 const bool ok = LIN ? x < xhi : fabsf(x - xmid) < xrad;
-                                const int n = ok ? (int)x : a.ns;
+                                const int n = ok ? (int)x : -2;
                                 const float f = __builtin_amdgcn_fractf(x);
                                 const float w = __builtin_amdgcn_sqrtf(x) * im[j];
                                 const float w1 = LIN ? w * f : 0.f;
@@ -259,18 +264,14 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                 // the rare jump (more than one sample, or back):
                                 // flush both running samples, start afresh
                                 const bool jump = !(same || next);
-                                if (__builtin_amdgcn_ballot_w64(jump)) {
-                                    if (jump) {
-                                        atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                        if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
-                                        c0 = 0.f;
-                                        c1 = 0.f;
-                                    }
+                                if (jump) {
+                                    atomicAdd(acc + adj_slot(nc), to_fixed(c0));
+                                    if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
+                                    c0 = 0.f;
+                                    c1 = 0.f;
                                 }
                                 // one sample on: sample nc is complete
-                                if (__builtin_amdgcn_ballot_w64(next)) {
-                                    if (next) atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                }
+                                if (next) atomicAdd(acc + adj_slot(nc), to_fixed(c0));
                                 c0 = (same ? c0 : (next ? c1 : 0.f)) + v0;
                                 c1 = (same ? c1 : 0.f) + v1;
                                 nc = n;
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const float inv_a = ldexpf(1.f, -Sa), inv_b = ldexpf(1.f, -Sb);
     for (int g = 0; g < ng; ++g) {
         float *out = a.traces + (size_t)s_k[g] * a.ns;
-        const unsigned long long *acc = s_acc + g * a.nsa;
+        const unsigned long long *acc = s_acc + g * a.nsa + kAdjFront;
         for (int n = tid; n < a.ns; n += NTH) {
             const float v = bad ? __int_as_float(0x7fc00000)
                                 : (from_fixed(acc[adj_slot(n)]) * inv_a) * inv_b;
