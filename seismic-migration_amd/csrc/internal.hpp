@@ -84,11 +84,11 @@ __device__ __forceinline__ float axis_f32(float start, float step, int i)
 } // namespace psktm
 
 // A migration plan: geometry binned into output tiles (see psktm_plan_create).
-// 64-bit slots per adjoint accumulator row for traces of ns samples: two
-// slots that take discarded contributions, then samples 0 .. ns-1 with one
-// pad slot every 16 samples (an upper bound; the kernel's layout is
-// kAdjFront + adj_slot(n)).
-inline int adj_row(int ns) { return (ns + 2) + ((ns + 2) >> 4) + 1; }
+// 64-bit slots per adjoint accumulator row for traces of ns samples: samples
+// 0 .. ns-1, then two slots that take discarded contributions (ns+1, ns+2),
+// rounded up to an odd count (rows of different traces start on different
+// LDS banks).
+inline int adj_row(int ns) { return (ns + 3) | 1; }
 
 struct psktm_plan {
     psktm_grid grid;

@@ -49,13 +49,9 @@ namespace {
 
 constexpr int kAdjMax = 8;    // traces per adjoint work item (upper bound)
 
-// LDS slot of trace sample n in an adjoint accumulator row: one pad slot after
-// every 16 samples, so that the regular sample strides of a wave's scatter do
-// not fall on the same banks (5% faster on a 2-D line than a plain row).
-__host__ __device__ inline int adj_slot(int n) { return n + (n >> 4); }
-// Row entries ahead of sample 0: adj_slot(-2) = -3 and adj_slot(-1) = -2 are
-// the discarded slots. kAdjFront + adj_slot(ns - 1) + 1 <= adj_row(ns).
-constexpr int kAdjFront = 3;
+// An adjoint accumulator row holds one 64-bit slot per trace sample,
+// n = 0 .. ns-1, and two more (ns+1, ns+2) that take the contributions of
+// invalid samples, which are never read (adj_row(ns) >= ns + 3).
 
 struct AdjArgs {
     int nx, ny, nt, ns;
@@ -64,6 +60,7 @@ struct AdjArgs {
     int cells_y, cell_mx, cell_my, win_x, win_y;
     float nsf, r2, wscale;
     float bound;                  // bound on (sum of |contributions| to one slot) / max|image*w|
+    float cbound;                 // bound on |a lane's merged contribution| / max|image*w|
     const unsigned *wmax;         // max |image * tau * sqrt(dt)| (bits), per launch
     int use_ap, interp, nsa, batch;
     const float4 *geo;
@@ -89,28 +86,37 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
                                           split, s_pairs, s_ab, s_mask);
 }
 
-// Exact fixed-point accumulation of the adjoint. A contribution y, already
-// multiplied by 2^S, is added as the 64-bit two's complement integer
-// rint(y); S is chosen per launch (from adj_max_kernel) so that no sum can
-// reach 2^53 in magnitude, so every rint(y) has |rint(y)| < 2^54 and the
-// integer sums never overflow. Integer addition is associative: the adjoint's
-// output does not depend on the order of the LDS atomics. No FP64 is used.
+// Exact fixed-point accumulation of the adjoint. A lane's merged
+// contribution y, already multiplied by 2^S, is rounded to the integer
+// nearest to it and added as a 64-bit two's complement integer: y splits
+// exactly into hf 2^24 + r (hf = rint(y / 2^24), |r| <= 2^23), and r is
+// rounded by one v_cvt_rpi_i32_f32 (ties up). S is chosen per launch (from
+// adj_max_kernel) so that every merged contribution has |y| < 2^53 and no
+// slot's sum can reach 2^62 in magnitude: the integer sums never overflow,
+// and integer addition is associative, so the adjoint's output does not
+// depend on the order of the LDS atomics. No FP64 is used.
 __device__ __forceinline__ unsigned long long to_fixed(float y)
 {
 #pragma clang fp contract(off)
-    const float yi = rintf(y);                                  // |yi| < 2^54
-    const float fh = floorf(yi * 5.9604644775390625e-08f);      // floor(yi / 2^24)
-    const unsigned lo = (unsigned)fmaf(-fh, 16777216.0f, yi);   // yi - fh 2^24, in [0, 2^24)
-    const int hi = (int)fh;                                     // |hi| < 2^30
-    const unsigned w0 = ((unsigned)hi << 24) | lo;
-    const unsigned w1 = (unsigned)(hi >> 8);                    // sign extension
-    return ((unsigned long long)w1 << 32) | w0;
+    const float hf = rintf(y * 5.9604644775390625e-08f);        // rint(y / 2^24), |hf| < 2^29
+    const float r = fmaf(-hf, 16777216.0f, y);                  // y - hf 2^24: exact
+#if defined(__HIP_DEVICE_COMPILE__)
+    int q;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(r));
+    // hf 2^24 + q in one 64-bit multiply-add
+    unsigned long long u;
+    asm("v_mad_i64_i32 %0, vcc, %1, %2, %3"
+        : "=v"(u) : "v"((int)hf), "s"(16777216), "v"((long long)q) : "vcc");
+    return u;
+#else
+    return (unsigned long long)((long long)(int)hf * 16777216ll + (long long)floorf(r + 0.5f));
+#endif
 }
 
-// The float nearest (within an ulp) to a fixed-point sum u (|u| < 2^53).
+// The float nearest (within an ulp) to a fixed-point sum u (|u| < 2^62).
 __device__ __forceinline__ float from_fixed(unsigned long long u)
 {
-    const int hi = (int)(unsigned)(u >> 32);                    // |hi| < 2^21: exact
+    const int hi = (int)(unsigned)(u >> 32);
     return fmaf((float)hi, 4294967296.0f, (float)(unsigned)u);
 }
 
@@ -141,6 +147,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     // fused), so the scatter weights do not depend on how the compiler
     // schedules the merged accumulation below.
 #pragma clang fp contract(off)
+    static_assert(TL % 4 == 0, "a lane's tau samples are loaded 4 at a time");
     extern __shared__ __align__(8) unsigned char s_raw[];
     unsigned long long *s_acc = reinterpret_cast<unsigned long long *>(s_raw);
     __shared__ float2 s_ab[kAdjMax][TB];
@@ -154,16 +161,19 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     for (int q = tid; q < a.batch * a.nsa; q += NTH) s_acc[q] = 0ull;
     if (tid < kAdjMax) s_k[tid] = (tid < ng) ? a.sorted_ids[p0 + tid] : -1;
 
-    // fixed-point scale 2^S: every sum stays below 2^53 (see to_fixed).
-    // mx * bound < 2^(e1 + e2) bounds the sum of |contributions| to a slot.
+    // fixed-point scale 2^S (see to_fixed): mx * cbound < 2^(e1 + e3) bounds
+    // a lane's merged contribution, mx * bound < 2^(e1 + e2) the sum of
+    // |contributions| to a slot
     const float mx = __uint_as_float(*a.wmax);
     const bool bad = !(mx <= 3.0e38f);           // NaN or inf in the image
-    int e1 = 0, e2 = 0;
+    int e1 = 0, e2 = 0, e3 = 0;
     (void)frexpf(bad ? 0.f : mx, &e1);
     (void)frexpf(a.bound, &e2);
+    (void)frexpf(a.cbound, &e3);
     // 2^S as two float factors: S reaches 250 for images near the FP32
     // denormal range (and is negative for images near FLT_MAX)
-    const int S = (bad || !(mx > 0.f)) ? 0 : min(250, max(-120, 53 - (e1 + e2)));
+    const int S = (bad || !(mx > 0.f)) ? 0
+                : min(250, max(-120, min(53 - (e1 + e3), 62 - (e1 + e2))));
     const int Sa = min(S, 120), Sb = S - Sa;
     const float scale_a = ldexpf(1.f, Sa), scale_b = ldexpf(1.f, Sb);
     __syncthreads();
@@ -172,11 +182,10 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const int ti0 = max(0, CI - a.cell_mx - a.win_x), ti1 = min(a.tiles_x - 1, CI - a.cell_mx + a.win_x);
     const int tj0 = max(0, CJ - a.cell_my - a.win_y), tj1 = min(a.tiles_y - 1, CJ - a.cell_my + a.win_y);
     // valid sample index: LINEAR 0 <= n <= ns-2, TRUNC 1 <= n <= ns-1
-    // (src/psktmCPU.c:82); anything else goes to sample -2: the discarded
-    // slots of samples -2 and -1 sit ahead of sample 0 in each row (row
-    // entry adj_slot(n) + kAdjFront; an inline constant, unlike ns)
+    // (src/psktmCPU.c:82)
     const float xhi = a.nsf - 1.f;
     const float xmid = 0.5f * (a.nsf + 1.f), xrad = 0.5f * (a.nsf - 1.f);
+    const int disc = a.ns + 1;
 
     for (int TI = ti0; TI <= ti1; ++TI) {
         for (int TJ = tj0; TJ <= tj1; ++TJ) {
@@ -215,24 +224,38 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                         wt[j] = ok ? (axis_f32(a.tau0, a.dtau, l) * a.wscale) : 0.f;
                         sl1[j] = (!V3 && ok) ? a.slow[l] : 0.f;
                     }
-#pragma unroll
+                    // bins and traces in rolled loops: the kernel's code
+                    // stays small (TL steps per trace and bin)
+#pragma unroll 1
                     for (int b = 0; b < TB; ++b) {
                         if (!((any >> b) & 1u)) continue;
                         const int i = i0 + b / a.tile_y, jj = j0 + b % a.tile_y;
+                        // the bin's image (and 3-D slowness) column through
+                        // buffer descriptors whose range is the column: the
+                        // lane's TL samples in 16-byte loads, zeros past nt
+                        const size_t col = ((size_t)i * a.ny + jj) * a.nt;
+                        const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc(
+                            (void *)(a.image + col), (short)0, a.nt * 4, 0x00020000);
+                        const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+                            (void *)(a.slow + (V3 ? col : 0)), (short)0, a.nt * 4, 0x00020000);
                         float im[TL], sl[TL];
 #pragma unroll
-                        for (int j = 0; j < TL; ++j) {
-                            const int l = lbase + j;
-                            const bool ok = l < a.nt;
-                            const size_t idx = ((size_t)i * a.ny + jj) * a.nt + l;
-                            im[j] = ok ? ((a.image[idx] * wt[j]) * scale_a) * scale_b : 0.f;
-                            sl[j] = V3 ? (ok ? a.slow[idx] : 0.f) : sl1[j];
+                        for (int q = 0; q < TL; q += 4) {
+                            const auto vi = __builtin_amdgcn_raw_buffer_load_b128(ri, 4 * (lbase + q), 0, 0);
+                            const auto vv = V3 ? __builtin_amdgcn_raw_buffer_load_b128(rv, 4 * (lbase + q), 0, 0)
+                                               : vi;
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                im[q + e] = ((__uint_as_float(vi[e]) * wt[q + e]) * scale_a) * scale_b;
+                                sl[q + e] = V3 ? __uint_as_float(vv[e]) : sl1[q + e];
+                            }
                         }
+#pragma unroll 1
                         for (int g = 0; g < kAdjMax; ++g) {
                             const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
                             if (!((m >> b) & 1u)) continue;
                             const float2 ab = s_ab[g][b];
-                            unsigned long long *acc = s_acc + g * a.nsa + kAdjFront;
+                            unsigned long long *acc = s_acc + g * a.nsa;
                             // A lane's TL consecutive tau samples reach few
                             // distinct trace samples (at most one more per tau
                             // step where the time map is flat or slow): hold
@@ -247,9 +270,10 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                 const float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
                                 // LINEAR: 0 <= x < ns-1 (x >= 0 holds); TRUNC:
                                 // 1 <= x < ns, as |x - (ns+1)/2| < (ns-1)/2. A
-                                // NaN time fails both and is discarded.
+                                // NaN time fails both. Invalid samples go to
+                                // the discarded slots ns+1, ns+2.
                                 const bool ok = LIN ? x < xhi : fabsf(x - xmid) < xrad;
-                                const int n = ok ? (int)x : -2;
+                                const int n = ok ? (int)x : disc;
                                 const float f = __builtin_amdgcn_fractf(x);
                                 const float w = __builtin_amdgcn_sqrtf(x) * im[j];
                                 const float w1 = LIN ? w * f : 0.f;
@@ -260,24 +284,20 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                     c1 = v1;
                                     continue;
                                 }
-                                const bool same = n == nc, next = n == nc + 1;
-                                // the rare jump (more than one sample, or back):
-                                // flush both running samples, start afresh
-                                const bool jump = !(same || next);
-                                if (jump) {
-                                    atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                    if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
-                                    c0 = 0.f;
-                                    c1 = 0.f;
+                                // moved on: the running sample nc is complete;
+                                // one sample on, nc + 1 becomes the running
+                                // sample, else (a jump) it is complete too
+                                const bool moved = n != nc, next = n == nc + 1;
+                                if (moved) {
+                                    atomicAdd(acc + nc, to_fixed(c0));
+                                    if (LIN && !next) atomicAdd(acc + nc + 1, to_fixed(c1));
                                 }
-                                // one sample on: sample nc is complete
-                                if (next) atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                                c0 = (same ? c0 : (next ? c1 : 0.f)) + v0;
-                                c1 = (same ? c1 : 0.f) + v1;
+                                c0 = (moved ? (next ? c1 : 0.f) : c0) + v0;
+                                c1 = (moved ? 0.f : c1) + v1;
                                 nc = n;
                             }
-                            atomicAdd(acc + adj_slot(nc), to_fixed(c0));
-                            if (LIN) atomicAdd(acc + adj_slot(nc + 1), to_fixed(c1));
+                            atomicAdd(acc + nc, to_fixed(c0));
+                            if (LIN) atomicAdd(acc + nc + 1, to_fixed(c1));
                         }
                     }
                 }
@@ -288,10 +308,10 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     const float inv_a = ldexpf(1.f, -Sa), inv_b = ldexpf(1.f, -Sb);
     for (int g = 0; g < ng; ++g) {
         float *out = a.traces + (size_t)s_k[g] * a.ns;
-        const unsigned long long *acc = s_acc + g * a.nsa + kAdjFront;
+        const unsigned long long *acc = s_acc + g * a.nsa;
         for (int n = tid; n < a.ns; n += NTH) {
             const float v = bad ? __int_as_float(0x7fc00000)
-                                : (from_fixed(acc[adj_slot(n)]) * inv_a) * inv_b;
+                                : (from_fixed(acc[n]) * inv_a) * inv_b;
             out[n] = out[n] + v;        // zeroed beforehand unless accumulating
         }
     }
@@ -553,10 +573,11 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
                        0, st, image, nimg, g.nt, g.tau0, g.dtau, wscale, p->adj_wmax);
     if ((s = check_launch("adj_max_kernel")) != PSKTM_OK) return s;
 
-    // 512 lanes x 4 consecutive tau samples: 16 waves per CU at two
-    // workgroups (LDS-bound by the FP64 rows); 256 x 8 kept 8 waves per CU
-    // and took 66.3 ms on the 2-D line against 48.9 ms, 1024 x 2 74.0 ms
-    constexpr int NTH = 512, TL = 4;
+    // 256 lanes x 8 consecutive tau samples, two traces per work item: four
+    // workgroups (16 waves) per CU on 2,000-sample traces. On the 512-CMP
+    // line: 31.8 ms; 512 x 4 with 4 traces 36.2 ms, 128 x 16 with one trace
+    // 32.6 ms.
+    constexpr int TL = 8, NTH = 256;
     // Without an aperture every work item visits every tile, so the adjoint
     // may take tiles of its own: 8-bin plan tiles are doubled along their
     // long axis (half the tile visits and barriers; 48.8 vs 49.7 ms on a
@@ -588,6 +609,8 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     // |contribution| <= sqrt(x) * max|image * w| with x < ns, and a slot sums
     // at most every update of its trace (<= nx * ny * nt)
     a.bound = (float)(1.01 * std::sqrt((double)p->n_samples) * (double)g.nx * g.ny * g.nt);
+    // a merged contribution adds at most TL of them
+    a.cbound = (float)(1.01 * std::sqrt((double)p->n_samples) * TL);
     a.wmax = p->adj_wmax;
     a.use_ap = !p->all_traces;
     a.interp = p->opt.interp;
