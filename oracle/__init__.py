@@ -34,7 +34,8 @@ def build() -> str:
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # PSKTM_ORACLE_LIBRARY: the host-sanitizer test run's instrumented build
+        path = os.environ.get("PSKTM_ORACLE_LIBRARY") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)

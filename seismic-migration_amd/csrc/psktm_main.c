@@ -125,6 +125,10 @@ int main(int argc, char **argv)
         return fail("no velModel.bin and no --v0");
     }
 
+    /* the config loader leaves positive sizes; their product must fit in
+     * size_t bytes (2^40 samples = 4 TiB, far past any host image) */
+    if ((double)X.imagePnts * (double)Y.imagePnts * (double)Z.imagePnts > 1099511627776.0)
+        return fail("image size out of range");
     const size_t nimg = (size_t)X.imagePnts * Y.imagePnts * Z.imagePnts;
     float *image = (float *)malloc(sizeof(float) * nimg);
     if (!image || !vel) return fail("out of host memory");

@@ -1,7 +1,9 @@
 """ctypes binding of libpsktm.so (the C ABI declared in include/psktm.h).
 
-The library is loaded from this directory only; there is no fallback. If it
-is missing, :func:`load` raises ImportError with the build command.
+The library is loaded from this directory only (or from PSKTM_LIBRARY, which
+the host-sanitizer test run sets to its instrumented build); there is no
+fallback. If it is missing, :func:`load` raises ImportError with the build
+command.
 """
 from __future__ import annotations
 
@@ -9,7 +11,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpsktm.so")
+LIB_PATH = os.environ.get("PSKTM_LIBRARY") or os.path.join(HERE, "libpsktm.so")
 
 _c_int, _c_float, _c_double, _p = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
 _c_size = ctypes.c_size_t
