@@ -170,12 +170,10 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     (void)frexpf(bad ? 0.f : mx, &e1);
     (void)frexpf(a.bound, &e2);
     (void)frexpf(a.cbound, &e3);
-    // 2^S as two float factors: S reaches 250 for images near the FP32
-    // denormal range (and is negative for images near FLT_MAX)
+    // S reaches 250 for images near the FP32 denormal range (and is negative
+    // for images near FLT_MAX): applied by ldexp, which rounds once
     const int S = (bad || !(mx > 0.f)) ? 0
                 : min(250, max(-120, min(53 - (e1 + e3), 62 - (e1 + e2))));
-    const int Sa = min(S, 120), Sb = S - Sa;
-    const float scale_a = ldexpf(1.f, Sa), scale_b = ldexpf(1.f, Sb);
     __syncthreads();
 
     const int CI = c / a.cells_y, CJ = c % a.cells_y;
@@ -246,7 +244,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                                : vi;
 #pragma unroll
                             for (int e = 0; e < 4; ++e) {
-                                im[q + e] = ((__uint_as_float(vi[e]) * wt[q + e]) * scale_a) * scale_b;
+                                im[q + e] = ldexpf(__uint_as_float(vi[e]) * wt[q + e], S);
                                 sl[q + e] = V3 ? __uint_as_float(vv[e]) : sl1[q + e];
                             }
                         }
@@ -305,13 +303,12 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
             __syncthreads();
         }
     }
-    const float inv_a = ldexpf(1.f, -Sa), inv_b = ldexpf(1.f, -Sb);
     for (int g = 0; g < ng; ++g) {
         float *out = a.traces + (size_t)s_k[g] * a.ns;
         const unsigned long long *acc = s_acc + g * a.nsa;
         for (int n = tid; n < a.ns; n += NTH) {
             const float v = bad ? __int_as_float(0x7fc00000)
-                                : (from_fixed(acc[n]) * inv_a) * inv_b;
+                                : ldexpf(from_fixed(acc[n]), -S);
             out[n] = out[n] + v;        // zeroed beforehand unless accumulating
         }
     }
