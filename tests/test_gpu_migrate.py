@@ -470,6 +470,35 @@ def test_half_pairs_stage_sizes_and_padding_bit_identical(cuda):
     assert imgs[0].abs().max() > 0
 
 
+def test_half_pairs_packed_fill_equals_entrywise_fill(cuda):
+    """The fill makes interior LINEAR chunks of 16-byte aligned rows with one
+    packed FP16 FMA per entry; rows that start off a 16-byte boundary take the
+    entry-by-entry path. The same FP16 values stored both ways (320-sample
+    rows at a 16-byte aligned base, then the same rows one sample further
+    into a buffer) give the same image bits, smooth traces and rows at the
+    ends of the FP16 range alike."""
+    import torch
+    grid, mid, off, tr, vel, dt, ap = small_3d(seed=50, ntr=1500, nx=9, ny=7, ns=320)
+    ntr, ns = tr.shape
+    assert (ns * 2) % 16 == 0
+    plan, t = gpu_plan(cuda, grid, mid, off, ns, dt, ap, splits=1)
+    q, sc = psktm.trace_gain(t(tr), dt, out_dtype=torch.float16)
+    rng = np.random.default_rng(50)
+    n = np.arange(ns)
+    q[1::4] = torch.as_tensor(np.where(n % 2 == 0, 65504.0, -65504.0).astype(np.float16)).to(cuda)
+    q[2::4] = torch.as_tensor(rng.uniform(-65504, 65504, (len(range(2, ntr, 4)), ns))
+                              .astype(np.float16)).to(cuda)
+    assert q.data_ptr() % 16 == 0
+    buf = torch.zeros(ntr * ns + 8, dtype=torch.float16, device=cuda)
+    shifted = buf[1:1 + ntr * ns].view(ntr, ns)
+    shifted.copy_(q)
+    assert shifted.data_ptr() % 16 == 2
+    a = psktm.migrate(plan, q, t(vel), scale=sc, gained=True)
+    b = psktm.migrate(plan, shifted, t(vel), scale=sc, gained=True)
+    assert torch.isfinite(a).all() and a.abs().max() > 0
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+
+
 def test_half_pairs_full_fp16_range(cuda):
     """Half pairs on samples at the ends of the FP16 range. The quantiser maps
     a trace alternating +1 / -0.9995 to +32768 / -32752, whose difference
