@@ -482,8 +482,8 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                 if (KH) {
                     // low-order part of (acc - cmp) * w: Fast2Sum residual of
                     // the difference (|cmp| <= |acc|) and the exact product
-                    // error, so that the FP64 reduction of the splits keeps
-                    // what each split's compensated sum carried
+                    // error, so that the compensated reduction of the splits
+                    // keeps what each split's compensated sum carried
                     const float lo = (acc[b][j] - sum) - cmp[KH ? b : 0][KH ? j : 0];
                     a.partial[(size_t)(a.n_split + split) * a.image_elems + idx] =
                         fmaf(sum, w, -v) + lo * w;

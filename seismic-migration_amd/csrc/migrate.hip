@@ -337,17 +337,26 @@ __global__ void zero_kernel(float *p, size_t n)
 
 // image[i] (+)= sum over splits of partial[s][i], in split order. Precise
 // mode (compensated != 0): partial also holds n_split low-order planes after
-// the n_split high-order ones, and everything is added in FP64 and rounded once.
+// the n_split high-order ones, and everything is added by compensated FP32
+// summation (each addition's rounding error recovered exactly by TwoSum and
+// summed separately, then added once: as accurate as summing in twice the
+// precision and rounding, with no FP64).
 __global__ void reduce_partials_kernel(const float *__restrict__ partial, int n_split,
                                        long long n, int accumulate, int compensated,
                                        float *__restrict__ image)
 {
+#pragma clang fp contract(off)
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (compensated) {
-        double t = accumulate ? (double)image[i] : 0.0;
-        for (int s = 0; s < 2 * n_split; ++s) t += (double)partial[(size_t)s * n + i];
-        image[i] = (float)t;
+        float t = accumulate ? image[i] : 0.f, e = 0.f;
+        for (int s = 0; s < 2 * n_split; ++s) {
+            const float x = partial[(size_t)s * n + i];
+            const float u = t + x, bv = u - t;
+            e += (t - (u - bv)) + (x - bv);
+            t = u;
+        }
+        image[i] = t + e;
         return;
     }
     float t = partial[i];
