@@ -187,8 +187,10 @@ typedef struct psktm_traces {
                                  The interpolated gained sample differs from
                                  the interpolated sample times sqrt(t) by
                                  f(1-f)(g[n+1]-g[n]) terms, ~1e-4 relative on
-                                 30 Hz data at 4 ms. psktm_demigrate applies
-                                 the per-update weight in either case. */
+                                 30 Hz data at 4 ms. psktm_demigrate is the
+                                 adjoint of the raw form,
+                                 psktm_demigrate_gained of the gained form;
+                                 precise mode takes either. */
 } psktm_traces;
 
 /* RMS velocity in m/s on the device: VEL_1D -> v[nt] (laterally invariant, as
@@ -263,6 +265,15 @@ psktm_status psktm_migrate(psktm_plan *plan, const psktm_traces *traces,
 psktm_status psktm_demigrate(psktm_plan *plan, const float *image,
                              const psktm_velocity *vel, float *traces_out,
                              int accumulate, void *stream);
+/* Adjoint of the gained operator: psktm_trace_gain (sample n times
+ * sqrtf(n*dt)) followed by psktm_migrate with gained = 1. Demigration with
+ * the weight tau per update (no sqrt(t)), each output sample then multiplied
+ * by sqrtf(n*dt): one v_sqrt_f32 fewer per update than psktm_demigrate. With
+ * it, least squares runs on the operator of the gained forward form. No
+ * counterpart in the reference (it has no adjoint). */
+psktm_status psktm_demigrate_gained(psktm_plan *plan, const float *image,
+                                    const psktm_velocity *vel, float *traces_out,
+                                    int accumulate, void *stream);
 
 /* Batched gathers: several independent migrations (e.g. offset classes),
  * each with its own traces and output patch, run as ONE persistent launch

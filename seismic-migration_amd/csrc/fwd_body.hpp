@@ -443,8 +443,9 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                     if (HP) {
                         acc[b][j] = fmaf(d, __int_as_float(sck), acc[b][j]);
                     } else if (KH) {
+                        // gained traces: no per-update weight
                         float &c = cmp[KH ? b : 0][KH ? j : 0];
-                        const float y = fmaf(d, __builtin_amdgcn_sqrtf(x), -c);
+                        const float y = GN ? d - c : fmaf(d, __builtin_amdgcn_sqrtf(x), -c);
                         const float t = acc[b][j] + y;
                         c = (t - acc[b][j]) - y;
                         acc[b][j] = t;

@@ -141,8 +141,8 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr,
                             const psktm_velocity &vel, float *image,
                             int accumulate, hipStream_t stream);
 psktm_status launch_demigrate(psktm_plan *p, const float *image,
-                              const psktm_velocity &vel, float *traces_out,
-                              int accumulate, hipStream_t stream);
+                              const psktm_velocity &vel, float *traces_out, int accumulate,
+                              int gained, hipStream_t st);
 
 // FP64 dot product in a fixed order (cg.hip): at most kDotPartials
 // workgroups write one partial each into `partial`, then one workgroup adds

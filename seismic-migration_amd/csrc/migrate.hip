@@ -61,6 +61,7 @@ struct AdjArgs {
     float nsf, r2, wscale;
     float bound;                  // bound on (sum of |contributions| to one slot) / max|image*w|
     float cbound;                 // bound on |a lane's merged contribution| / max|image*w|
+    float gain_dt;                // gained operator: output sample n times sqrtf(n gain_dt); 0: none
     const unsigned *wmax;         // max |image * tau * sqrt(dt)| (bits), per launch
     int use_ap, interp, nsa, batch;
     const float4 *geo;
@@ -140,7 +141,7 @@ __global__ void adj_max_kernel(const float *__restrict__ image, long long n, int
     if ((threadIdx.x & 63) == 0) atomicMax(out, m != m ? 0x7fc00000u : __float_as_uint(m));
 }
 
-template <int NTH, int TB, int TL, bool V3, bool LIN>
+template <int NTH, int TB, int TL, bool V3, bool LIN, bool GN>
 __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
 {
     // Products and sums round one at a time (explicit fmaf calls stay
@@ -273,7 +274,8 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                 const bool ok = LIN ? x < xhi : fabsf(x - xmid) < xrad;
                                 const int n = ok ? (int)x : disc;
                                 const float f = __builtin_amdgcn_fractf(x);
-                                const float w = __builtin_amdgcn_sqrtf(x) * im[j];
+                                // gained operator: weight tau alone (in im)
+                                const float w = GN ? im[j] : __builtin_amdgcn_sqrtf(x) * im[j];
                                 const float w1 = LIN ? w * f : 0.f;
                                 const float v0 = w - w1, v1 = w1;
                                 if (j == 0) {
@@ -307,8 +309,10 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
         float *out = a.traces + (size_t)s_k[g] * a.ns;
         const unsigned long long *acc = s_acc + g * a.nsa;
         for (int n = tid; n < a.ns; n += NTH) {
-            const float v = bad ? __int_as_float(0x7fc00000)
-                                : ldexpf(from_fixed(acc[n]), -S);
+            float v = bad ? __int_as_float(0x7fc00000) : ldexpf(from_fixed(acc[n]), -S);
+            // the gained operator's trace gain (psktm_trace_gain's sqrtf(n dt)),
+            // transposed: applied to the output sample
+            if (GN) v = v * sqrtf((float)n * a.gain_dt);
             out[n] = out[n] + v;        // zeroed beforehand unless accumulating
         }
     }
@@ -372,7 +376,7 @@ using AdjFn = void (*)(AdjArgs);
 template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP>
 FwdFn fwd_pick()
 {
-    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH) && !(KH && GN) && (GN || !HP))
+    if constexpr (acc_regs(TB, TL, V3, KH) <= reg_budget(KH) && !(KH && HP) && (GN || !HP))
         return migrate_fwd_kernel<NTH, TB, TL, V3, KH, GN, HP>;
     else
         return nullptr;
@@ -392,16 +396,18 @@ FwdFn fwd_by_tl(int tl)
 }
 
 // mode: 0 raw traces, 1 precise (compensated), 2 gained traces, 3 gained
-// FP16 traces staged as half pairs
+// FP16 traces staged as half pairs, 4 precise with gained traces
 template <int NTH, int TB>
 FwdFn fwd_by_flags(int tl, bool v3, int mode)
 {
     if (v3) {
+        if (mode == 4) return fwd_by_tl<NTH, TB, true, true, true>(tl);
         if (mode == 1) return fwd_by_tl<NTH, TB, true, true, false>(tl);
         if (mode == 2) return fwd_by_tl<NTH, TB, true, false, true>(tl);
         if (mode == 3) return fwd_by_tl<NTH, TB, true, false, true, true>(tl);
         return fwd_by_tl<NTH, TB, true, false, false>(tl);
     }
+    if (mode == 4) return fwd_by_tl<NTH, TB, false, true, true>(tl);
     if (mode == 1) return fwd_by_tl<NTH, TB, false, true, false>(tl);
     if (mode == 2) return fwd_by_tl<NTH, TB, false, false, true>(tl);
     if (mode == 3) return fwd_by_tl<NTH, TB, false, false, true, true>(tl);
@@ -410,7 +416,7 @@ FwdFn fwd_by_flags(int tl, bool v3, int mode)
 
 FwdFn fwd_kernel(int nth, int tb, int tl, bool v3, bool kh, bool gn, bool hp)
 {
-    const int mode = kh ? 1 : (gn ? (hp ? 3 : 2) : 0);
+    const int mode = kh ? (gn ? 4 : 1) : (gn ? (hp ? 3 : 2) : 0);
     if (nth == 256) return tb == 8 ? fwd_by_flags<256, 8>(tl, v3, mode) : fwd_by_flags<256, 16>(tl, v3, mode);
     return tb == 8 ? fwd_by_flags<512, 8>(tl, v3, mode) : fwd_by_flags<512, 16>(tl, v3, mode);
 }
@@ -460,15 +466,14 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     PSKTM_CHECK(tr.dtype == PSKTM_F32 || p->n_traces == 0 || tr.scale, PSKTM_E_INVAL,
                 "16-bit traces need per-trace scales");
     PSKTM_CHECK(tr.gained == 0 || tr.gained == 1, PSKTM_E_INVAL, "gained must be 0 or 1");
-    PSKTM_CHECK(!(tr.gained && p->opt.precise), PSKTM_E_UNSUPPORTED,
-                "precise mode takes raw traces (gained = 0)");
     const psktm_grid &g = p->grid;
     const int tb = p->tile_x * p->tile_y;
     const bool v3 = vel.kind == PSKTM_VEL_3D;
     const bool kh = p->opt.precise != 0;
     const bool gn = tr.gained != 0;
-    // gained FP16 traces: half pairs in LDS (half the bytes per staged row)
-    const bool hp = gn && tr.dtype == PSKTM_F16;
+    // gained FP16 traces: half pairs in LDS (half the bytes per staged row);
+    // precise mode stages FP32 pairs
+    const bool hp = gn && !kh && tr.dtype == PSKTM_F16;
     PSKTM_CHECK(!hp || ((uintptr_t)tr.data & 1u) == 0, PSKTM_E_INVAL,
                 "FP16 trace data must be 2-byte aligned");
     FwdLaunch L{};
@@ -550,7 +555,7 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
 }
 
 psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_velocity &vel,
-                              float *traces_out, int accumulate, hipStream_t st)
+                              float *traces_out, int accumulate, int gained, hipStream_t st)
 {
     psktm_status s;
     if ((s = validate_velocity(vel)) != PSKTM_OK) return s;
@@ -572,7 +577,9 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     if (p->n_adj_items == 0) return check_launch("zero_kernel");
     // the fixed-point scale: max |image * tau * sqrt(dt)|, stream-ordered
     const long long nimg = (long long)g.nx * g.ny * g.nt;
-    const float wscale = (float)std::sqrt((double)p->dt);
+    // weight tau*sqrt(dt) per update (times sqrt(x)), or tau alone for the
+    // gained operator (its sqrt(n dt) acts on the output samples)
+    const float wscale = gained ? 1.0f : (float)std::sqrt((double)p->dt);
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, (float *)p->adj_wmax, (size_t)1);
     hipLaunchKernelGGL(adj_max_kernel,
                        dim3((unsigned)std::min<long long>((nimg + 255) / 256, 2048)), dim3(256),
@@ -595,12 +602,17 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     const int atb = atx * aty;
     const bool lin = p->opt.interp == PSKTM_INTERP_LINEAR;
     AdjFn fn;
-    if (atb == 8)
-        fn = v3 ? (lin ? demigrate_kernel<NTH, 8, TL, true, true> : demigrate_kernel<NTH, 8, TL, true, false>)
-                : (lin ? demigrate_kernel<NTH, 8, TL, false, true> : demigrate_kernel<NTH, 8, TL, false, false>);
-    else
-        fn = v3 ? (lin ? demigrate_kernel<NTH, 16, TL, true, true> : demigrate_kernel<NTH, 16, TL, true, false>)
-                : (lin ? demigrate_kernel<NTH, 16, TL, false, true> : demigrate_kernel<NTH, 16, TL, false, false>);
+#define PSKTM_ADJ_PICK(TBV)                                                                  \
+    (gained ? (v3 ? (lin ? demigrate_kernel<NTH, TBV, TL, true, true, true>                  \
+                         : demigrate_kernel<NTH, TBV, TL, true, false, true>)                \
+                  : (lin ? demigrate_kernel<NTH, TBV, TL, false, true, true>                 \
+                         : demigrate_kernel<NTH, TBV, TL, false, false, true>))              \
+            : (v3 ? (lin ? demigrate_kernel<NTH, TBV, TL, true, true, false>                 \
+                         : demigrate_kernel<NTH, TBV, TL, true, false, false>)               \
+                  : (lin ? demigrate_kernel<NTH, TBV, TL, false, true, false>                \
+                         : demigrate_kernel<NTH, TBV, TL, false, false, false>)))
+    fn = atb == 8 ? PSKTM_ADJ_PICK(8) : PSKTM_ADJ_PICK(16);
+#undef PSKTM_ADJ_PICK
     AdjArgs a{};
     a.nx = g.nx; a.ny = g.ny; a.nt = g.nt; a.ns = p->n_samples;
     a.x0 = g.x0; a.dx = g.dx; a.y0 = g.y0; a.dy = g.dy; a.tau0 = g.tau0; a.dtau = g.dtau;
@@ -617,6 +629,7 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
     a.bound = (float)(1.01 * std::sqrt((double)p->n_samples) * (double)g.nx * g.ny * g.nt);
     // a merged contribution adds at most TL of them
     a.cbound = (float)(1.01 * std::sqrt((double)p->n_samples) * TL);
+    a.gain_dt = gained ? p->dt : 0.0f;
     a.wmax = p->adj_wmax;
     a.use_ap = !p->all_traces;
     a.interp = p->opt.interp;
@@ -650,7 +663,17 @@ psktm_status psktm_demigrate(psktm_plan *plan, const float *image, const psktm_v
     psktm__clear_error();
     PSKTM_CHECK(plan && image && vel && (traces_out || plan->n_traces == 0), PSKTM_E_INVAL,
                 "NULL argument");
-    return launch_demigrate(plan, image, *vel, traces_out, accumulate, as_stream(stream));
+    return launch_demigrate(plan, image, *vel, traces_out, accumulate, 0, as_stream(stream));
+}
+
+psktm_status psktm_demigrate_gained(psktm_plan *plan, const float *image,
+                                    const psktm_velocity *vel, float *traces_out, int accumulate,
+                                    void *stream)
+{
+    psktm__clear_error();
+    PSKTM_CHECK(plan && image && vel && (traces_out || plan->n_traces == 0), PSKTM_E_INVAL,
+                "NULL argument");
+    return launch_demigrate(plan, image, *vel, traces_out, accumulate, 1, as_stream(stream));
 }
 
 } // extern "C"

@@ -107,6 +107,7 @@ _SIGNATURES = {
     "psktm_plan_bytes": (_c_int, [_p, _P(_c_size)]),
     "psktm_migrate": (_c_int, [_p, _P(PsktmTraces), _P(PsktmVelocity), _p, _c_int, _p]),
     "psktm_demigrate": (_c_int, [_p, _p, _P(PsktmVelocity), _p, _c_int, _p]),
+    "psktm_demigrate_gained": (_c_int, [_p, _p, _P(PsktmVelocity), _p, _c_int, _p]),
     "psktm_batch_create": (_c_int, [_P(_p), _P(PsktmGather), _c_int, _P(PsktmSurvey),
                                     _P(PsktmOptions), _p]),
     "psktm_batch_destroy": (_c_int, [_p]),

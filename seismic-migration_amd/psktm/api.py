@@ -266,8 +266,14 @@ def migrate(plan: Plan, traces, vel, image=None, *, scale=None, dtype=None, accu
     return image
 
 
-def demigrate(plan: Plan, image, vel, traces=None, *, accumulate=False, stream=None):
-    """Adjoint of :func:`migrate`: image -> float32 traces [n_traces, n_samples]."""
+def demigrate(plan: Plan, image, vel, traces=None, *, accumulate=False, gained=False,
+              stream=None):
+    """Adjoint of :func:`migrate`: image -> float32 traces [n_traces, n_samples].
+
+    ``gained``: the adjoint of the gained operator (:func:`trace_gain`, then
+    :func:`migrate` with ``gained=True``): weight tau per update, output
+    sample n times sqrtf(n dt).
+    """
     torch = _torch()
     L = _lib.load()
     s, g = plan.survey, plan.grid
@@ -276,8 +282,9 @@ def demigrate(plan: Plan, image, vel, traces=None, *, accumulate=False, stream=N
     if traces is None:
         traces = torch.empty((s.n_traces, s.n_samples), dtype=torch.float32, device=image.device)
     _need(traces, torch.float32, "traces", s.n_traces * s.n_samples)
-    check(L.psktm_demigrate(plan.handle, _p(_ptr(image)), ctypes.byref(v), _p(_ptr(traces)),
-                            int(bool(accumulate)), _stream(stream)))
+    fn = L.psktm_demigrate_gained if gained else L.psktm_demigrate
+    check(fn(plan.handle, _p(_ptr(image)), ctypes.byref(v), _p(_ptr(traces)),
+             int(bool(accumulate)), _stream(stream)))
     return traces
 
 
@@ -306,16 +313,20 @@ def rho_filter(traces, dw_us: float, opt: int = 2, out_dtype=None, *, matlab_tra
     return out if scale is None else (out, scale)
 
 
-def trace_gain(traces, dt: float, out_dtype=None, *, stream=None):
+def trace_gain(traces, dt: float, out_dtype=None, *, out=None, stream=None):
     """sample n times sqrtf(n dt) (the trace-domain part of the weight), as
-    float32, or (quantised traces, per-trace scale) for float16 / bfloat16."""
+    float32 (into ``out`` if given), or (quantised traces, per-trace scale)
+    for float16 / bfloat16."""
     torch = _torch()
     L = _lib.load()
     _need(traces, torch.float32, "traces")
     n, m = traces.shape
     out_dtype = torch.float32 if out_dtype is None else out_dtype
     kinds = {torch.float32: _lib.F32, torch.float16: _lib.F16, torch.bfloat16: _lib.BF16}
-    out = torch.empty((n, m), dtype=out_dtype, device=traces.device)
+    if out is None or out_dtype != torch.float32:
+        out = torch.empty((n, m), dtype=out_dtype, device=traces.device)
+    else:
+        _need(out, torch.float32, "out", n * m)
     scale = torch.empty((n,), dtype=torch.float32, device=traces.device) \
         if out_dtype != torch.float32 else None
     check(L.psktm_trace_gain(_p(_ptr(traces)), int(n), int(m), float(dt), kinds[out_dtype],

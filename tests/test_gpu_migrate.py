@@ -362,12 +362,18 @@ def test_gained_traces_vs_fp64(cuda, interp):
         assert e16 <= 1e-3
 
 
-def test_gained_precise_unsupported_and_batch(cuda):
-    import torch
+def test_gained_flag_checked(cuda):
+    """The trace form's gained flag must be 0 or 1 (precise mode takes both)."""
+    import ctypes
     grid, mid, off, tr, vel, dt, ap = small_3d(seed=44, ntr=500, nx=5, ny=5)
     plan, t = gpu_plan(cuda, grid, mid, off, tr.shape[1], dt, ap, precise=True)
-    with pytest.raises(psktm.PsktmError):
-        psktm.migrate(plan, t(tr), t(vel), gained=True)
+    L = psktm._lib.load()
+    d, v = t(tr), t(vel)
+    img = psktm.migrate(plan, d, v, gained=True)
+    trs = psktm._lib.PsktmTraces(psktm._lib.F32, d.data_ptr(), None, 2)
+    vv = psktm._lib.PsktmVelocity(psktm._lib.VEL_3D, v.data_ptr())
+    st = L.psktm_migrate(plan.handle, ctypes.byref(trs), ctypes.byref(vv), img.data_ptr(), 0, None)
+    assert st == psktm._lib.E_INVAL
 
 
 @pytest.mark.parametrize("axis", ["x", "y"])
