@@ -89,6 +89,10 @@ __device__ __forceinline__ float axis_f32(float start, float step, int i)
 // rounded up to an odd count (rows of different traces start on different
 // LDS banks).
 inline int adj_row(int ns) { return (ns + 3) | 1; }
+// Traces per adjoint work item (upper bound): one. Eight 256-lane workgroups
+// per CU on 2,000-sample traces; on the 512-CMP line 2, 3 and 4 traces per
+// item were 3%, 11% and 40% slower.
+constexpr int kAdjMax = 1;
 
 struct psktm_plan {
     psktm_grid grid;

@@ -47,7 +47,6 @@
 namespace psktm {
 namespace {
 
-constexpr int kAdjMax = 8;    // traces per adjoint work item (upper bound)
 
 // An adjoint accumulator row holds one 64-bit slot per trace sample,
 // n = 0 .. ns-1, and two more (ns+1, ns+2) that take the contributions of
@@ -586,10 +585,10 @@ psktm_status launch_demigrate(psktm_plan *p, const float *image, const psktm_vel
                        0, st, image, nimg, g.nt, g.tau0, g.dtau, wscale, p->adj_wmax);
     if ((s = check_launch("adj_max_kernel")) != PSKTM_OK) return s;
 
-    // 256 lanes x 8 consecutive tau samples, two traces per work item: four
-    // workgroups (16 waves) per CU on 2,000-sample traces. On the 512-CMP
-    // line: 31.8 ms; 512 x 4 with 4 traces 36.2 ms, 128 x 16 with one trace
-    // 32.6 ms.
+    // 256 lanes x 8 consecutive tau samples, one trace per work item: eight
+    // workgroups (32 waves) per CU on 2,000-sample traces. On the 512-CMP
+    // line: 31.0 ms; two traces per item 31.8-32.1 ms, 512 x 4 with four
+    // traces 36.2 ms, 128 x 16 with one trace 32.6 ms.
     constexpr int TL = 8, NTH = 256;
     // Without an aperture every work item visits every tile, so the adjoint
     // may take tiles of its own: 8-bin plan tiles are doubled along their

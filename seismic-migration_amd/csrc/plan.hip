@@ -421,10 +421,9 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
     }
 
     // ---- adjoint work items: (cell, first sorted position) ----------------
-    // traces per adjoint work item: up to 2 (four 256-lane workgroups per CU
-    // on 2,000-sample traces; 3 and 4 traces were 8% and 37% slower on the
-    // 512-CMP line), as many as 64-bit accumulator rows fit in 150 KB of LDS
-    p->adj_batch = (int)std::max(1ll, std::min(2ll, (150ll * 1024) /
+    // traces per adjoint work item: kAdjMax, as long as their 64-bit
+    // accumulator rows fit in 150 KB of LDS
+    p->adj_batch = (int)std::max(1ll, std::min((long long)kAdjMax, (150ll * 1024) /
                                                         (8ll * adj_row(s.n_samples))));
     std::vector<int> items;
     for (int c = 0; c < p->n_cells; ++c)
