@@ -322,6 +322,15 @@ __device__ __forceinline__ void dft_regs(float2 (&v)[16])
     for (int k = 0; k < R; ++k) v[k] = t[k];
 }
 
+// Transforms of up to 2048 points keep the filter H[k] (M float2) and the
+// per-sample gain (M floats) in LDS tables instead of evaluating them per
+// transform (two 256-lane workgroups per CU still fit: 75 KB at M = 2048).
+__host__ __device__ constexpr bool rho_tables(int M) { return M <= 2048; }
+__host__ __device__ constexpr size_t rho_table_bytes(int M)
+{
+    return rho_tables(M) ? (size_t)M * (sizeof(float2) + sizeof(float)) : 0;
+}
+
 // LDS index with one pad slot per 16 entries (the 16-apart writes of the
 // radix-16 passes would otherwise fall on one bank).
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
@@ -383,17 +392,29 @@ void rho_fast_kernel(
 #pragma clang fp contract(off)
     constexpr int L = M / 16, FPW = kRhoThreads / L, R3 = M / 256;
     constexpr int BUF = M + M / 16;
+    constexpr bool TAB = rho_tables(M);
     extern __shared__ float2 smem[];
     float2 *tw = smem;
     __shared__ float s_red[2][kRhoThreads / 64];
     const int tid = threadIdx.x, slot = tid / L, j = tid % L;
     float2 *buf = smem + M + slot * BUF;
-    for (int q = tid; q < M; q += kRhoThreads) tw[q] = twg[q];
+    // TAB: the filter H[k] and the gain of every sample, once per workgroup
+    float2 *hk = smem + M + FPW * BUF;
+    float *gtab = reinterpret_cast<float *>(hk + M);
+    for (int q = tid; q < M; q += kRhoThreads) {
+        tw[q] = twg[q];
+        if (TAB) {
+            hk[q] = rho_response(q, M, wscale, opt);
+            gtab[q] = row_gain(q, gain_dt);
+        }
+    }
     const int rot = (shift && (n & 1)) ? 1 : 0;
     const float invM = 1.0f / (float)M;
     const int pairs = (n + 1) / 2;
-    for (int p0 = blockIdx.x * FPW; p0 < pairs; p0 += gridDim.x * FPW) {
-        const int pr = p0 + slot;
+    // the two rows of transform `slot` of the group at p, into registers
+    // (zero past m and for rows past n); lane j takes samples j + L r
+    auto load_group = [&](int p, float2 (&regs)[16]) {
+        const int pr = p + slot;
         const bool live = pr < pairs;
         const int ra = 2 * pr, rb = ra + 1;
         const bool has_b = live && rb < n;
@@ -403,7 +424,6 @@ void rho_fast_kernel(
         const int ia = __builtin_amdgcn_readfirstlane(live ? (ra + rot) % n : 0);
         const int ib = __builtin_amdgcn_readfirstlane(has_b ? (rb + rot) % n : 0);
         const float *rowa = in + (size_t)ia * m, *rowb = in + (size_t)ib * m;
-        float2 regs[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = j + L * r;
@@ -414,6 +434,15 @@ void rho_fast_kernel(
             }
             regs[r] = make_float2(va, vb);
         }
+    };
+    const int stride = gridDim.x * FPW;
+    float2 regs[16];
+    if (blockIdx.x * FPW < pairs) load_group(blockIdx.x * FPW, regs);
+    for (int p0 = blockIdx.x * FPW; p0 < pairs; p0 += stride) {
+        const int pr = p0 + slot;
+        const bool live = pr < pairs;
+        const int ra = 2 * pr, rb = ra + 1;
+        const bool has_b = live && rb < n;
         __syncthreads();                       // previous group done with buf (and tw loaded)
         stockham_pass<M, 16, 1, false, true, false>(regs, nullptr, buf, tw, j);
         __syncthreads();
@@ -424,7 +453,8 @@ void rho_fast_kernel(
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int k = j + L * r;
-            regs[r] = k == 0 ? make_float2(0.f, 0.f) : cmul(rho_response(k, M, wscale, opt), regs[r]);
+            regs[r] = k == 0 ? make_float2(0.f, 0.f)
+                             : cmul(TAB ? hk[k] : rho_response(k, M, wscale, opt), regs[r]);
         }
         __syncthreads();                       // pass-3 reads done before buf is rewritten
         stockham_pass<M, 16, 1, true, true, false>(regs, nullptr, buf, tw, j);
@@ -441,7 +471,7 @@ void rho_fast_kernel(
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = j + L * r;
-            const float gq = row_gain(q, gain_dt);
+            const float gq = TAB ? gtab[q] : row_gain(q, gain_dt);
             const float ya = (regs[r].x * invM) * gq, yb = (regs[r].y * invM) * gq;
             buf[pidx(q)] = make_float2(ya, yb);
             if (q < m) {
@@ -449,6 +479,9 @@ void rho_fast_kernel(
                 mb = fmaxf(mb, fabsf(yb));
             }
         }
+        // the registers are free: the next group's loads go out now and land
+        // while this group's scale reduction and stores run
+        if (p0 + stride < pairs) load_group(p0 + stride, regs);
         float inv_a = 1.f, inv_b = 1.f;
         if (DT != PSKTM_F32) {
             // max over the L lanes of this transform: waves, then LDS
@@ -585,7 +618,8 @@ psktm_status psktm_rho_filter_gain(const float *in, int n_traces, int n_samples,
                        : M == 2048 ? pick(std::integral_constant<int, 2048>{})
                                    : pick(std::integral_constant<int, 4096>{});
         const int L = M / 16, fpw = kRhoThreads / L;
-        const size_t lds = sizeof(float2) * ((size_t)M + (size_t)fpw * (M + M / 16));
+        const size_t lds = sizeof(float2) * ((size_t)M + (size_t)fpw * (M + M / 16)) +
+                           rho_table_bytes(M);
         { psktm_status s_; if ((s_ = ensure_dynamic_lds(fn, lds)) != PSKTM_OK) return s_; }
         int cus = 0;
         PSKTM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
