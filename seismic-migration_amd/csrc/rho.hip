@@ -424,15 +424,20 @@ void rho_fast_kernel(
         const int ia = __builtin_amdgcn_readfirstlane(live ? (ra + rot) % n : 0);
         const int ib = __builtin_amdgcn_readfirstlane(has_b ? (rb + rot) % n : 0);
         const float *rowa = in + (size_t)ia * m, *rowb = in + (size_t)ib * m;
+        // buffer descriptors whose range is the row (none for a row past n):
+        // loads past it return zeros, so the zero padding needs no branches
+        const int na = __builtin_amdgcn_readfirstlane(live ? m * 4 : 0);
+        const int nb = __builtin_amdgcn_readfirstlane(has_b ? m * 4 : 0);
+        const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)rowa, (short)0, na, 0x00020000);
+        const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)rowb, (short)0, nb, 0x00020000);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int q = j + L * r;
-            float va = 0.f, vb = 0.f;
-            if (live && q < m) {
-                va = rowa[q];
-                if (has_b) vb = rowb[q];
-            }
-            regs[r] = make_float2(va, vb);
+            regs[r] = make_float2(
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsa, 4 * q, 0, 0)),
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsb, 4 * q, 0, 0)));
         }
     };
     const int stride = gridDim.x * FPW;
