@@ -110,6 +110,18 @@ __device__ __forceinline__ void store_q(void *out, int dtype, size_t idx, float 
     else static_cast<unsigned short *>(out)[idx] = f2bf16_rne(y * inv);
 }
 
+// The 16-bit code store_q writes for y (FP16 or BF16), for paired stores.
+template <int DT>
+__device__ __forceinline__ unsigned q16(float y, float inv)
+{
+    if (DT == PSKTM_F16) {
+        float p = y * inv;
+        asm volatile("" : "+v"(p));
+        return __half_as_ushort(__float2half_rn(p));
+    }
+    return f2bf16_rne(y * inv);
+}
+
 // sqrtf(n * gain_dt), or 1 without a gain (gain_dt == 0); one rounding for the
 // product, correctly rounded sqrt.
 __device__ __forceinline__ float row_gain(int n, float gain_dt)
@@ -441,6 +453,7 @@ void rho_fast_kernel(
         }
     };
     const int stride = gridDim.x * FPW;
+    const bool paired = (m & 1) == 0 && ((size_t)out & 7) == 0;
     float2 regs[16];
     if (blockIdx.x * FPW < pairs) load_group(blockIdx.x * FPW, regs);
     for (int p0 = blockIdx.x * FPW; p0 < pairs; p0 += stride) {
@@ -520,10 +533,33 @@ void rho_fast_kernel(
         if (live) {
             const size_t oa = (size_t)__builtin_amdgcn_readfirstlane(ra) * m;
             const size_t ob = oa + m;
-            for (int q = j; q < m; q += L) {
-                const float2 y = buf[pidx(q)];
-                store_q(out, DT, oa + q, y.x, inv_a);
-                if (has_b) store_q(out, DT, ob + q, y.y, inv_b);
+            if (paired) {
+                // even m (and an 8-byte aligned output): every row starts on
+                // an even sample, so each lane stores samples q and q + 1 of
+                // both traces as one 4- (FP16/BF16) or 8-byte (FP32) word;
+                // pidx keeps an even q and q + 1 adjacent in buf
+                for (int q = 2 * j; q < m; q += 2 * L) {
+                    const int e = pidx(q);
+                    const float2 y0 = buf[e], y1 = buf[e + 1];
+                    if (DT == PSKTM_F32) {
+                        float *o = static_cast<float *>(out);
+                        *reinterpret_cast<float2 *>(o + oa + q) = make_float2(y0.x, y1.x);
+                        if (has_b) *reinterpret_cast<float2 *>(o + ob + q) = make_float2(y0.y, y1.y);
+                    } else {
+                        unsigned short *o = static_cast<unsigned short *>(out);
+                        *reinterpret_cast<unsigned *>(o + oa + q) =
+                            q16<DT>(y0.x, inv_a) | (q16<DT>(y1.x, inv_a) << 16);
+                        if (has_b)
+                            *reinterpret_cast<unsigned *>(o + ob + q) =
+                                q16<DT>(y0.y, inv_b) | (q16<DT>(y1.y, inv_b) << 16);
+                    }
+                }
+            } else {
+                for (int q = j; q < m; q += L) {
+                    const float2 y = buf[pidx(q)];
+                    store_q(out, DT, oa + q, y.x, inv_a);
+                    if (has_b) store_q(out, DT, ob + q, y.y, inv_b);
+                }
             }
         }
     }

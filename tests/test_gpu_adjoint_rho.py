@@ -254,12 +254,16 @@ def test_rho_closed_form_cosine_on_gpu(cuda):
     np.testing.assert_allclose(got[1], expect, atol=2e-5 * np.sqrt(M * w0))
 
 
+@pytest.mark.parametrize("m", [1500, 1499, 700])
 @pytest.mark.parametrize("kind,tol", [("f16", 2 ** -11), ("bf16", 2 ** -8)])
-def test_rho_fused_quantisation(cuda, kind, tol):
+def test_rho_fused_quantisation(cuda, kind, tol, m):
+    # even m takes the kernel's paired (two samples per lane) stores, odd m
+    # the per-sample ones; both must give the codes of the separate
+    # quantisation of the FP32 output, bit for bit, with and without the gain
     import torch
     dt = torch.float16 if kind == "f16" else torch.bfloat16
     rng = np.random.default_rng(5)
-    x = (rng.standard_normal((7, 1500)) * rng.uniform(1e-3, 1e3, (7, 1))).astype(np.float32)
+    x = (rng.standard_normal((7, m)) * rng.uniform(1e-3, 1e3, (7, 1))).astype(np.float32)
     xt = torch.as_tensor(x).to(cuda)
     f32 = psktm.rho_filter(xt, 4000.0, 2)
     q, sc = psktm.rho_filter(xt, 4000.0, 2, out_dtype=dt)
@@ -268,8 +272,13 @@ def test_rho_fused_quantisation(cuda, kind, tol):
     assert err.max().item() <= tol
     q2, sc2 = psktm.quantize(f32, dt)
     assert torch.equal(sc2, sc)
+    assert torch.equal(q2.view(torch.int16), q.view(torch.int16))
     qmax = 32768.0 if kind == "f16" else 1.0
     assert torch.allclose(sc, f32.abs().amax(1) / qmax, rtol=1e-6)
+    qg, scg = psktm.rho_filter(xt, 4000.0, 2, out_dtype=dt, gain=True)
+    qg2, scg2 = psktm.trace_gain(f32, 4000.0 * 1e-6, dt)
+    assert torch.equal(scg2, scg)
+    assert torch.equal(qg2.view(torch.int16), qg.view(torch.int16))
 
 
 def test_quantize_zero_trace(cuda):
