@@ -343,6 +343,22 @@ __host__ __device__ constexpr size_t rho_table_bytes(int M)
     return rho_tables(M) ? (size_t)M * (sizeof(float2) + sizeof(float)) : 0;
 }
 
+// Twiddles of the radix-16 kernel in the order its lanes read them: for the
+// Ns = 16 pass W_M^(r k M/256) at (r - 1) 16 + k (0 < r < 16, k < 16), then
+// for the Ns = 256 pass W_M^(r k) at 240 + (r - 1) 256 + k (0 < r < M/256,
+// k < 256). Lanes on consecutive k read consecutive entries, free of bank
+// conflicts (as W_M^e the Ns = 16 pass's reads fell 16 k dwords apart, on
+// two banks).
+__host__ __device__ constexpr bool rho_fast_length(int M)
+{
+    return M == 1024 || M == 2048 || M == 4096;
+}
+__host__ __device__ constexpr int rho_fast_tw_count(int M) { return 240 + (M / 256 - 1) * 256; }
+__host__ __device__ constexpr int rho_fast_tw_index(int Ns, int r, int k)
+{
+    return (Ns == 16 ? 0 : 240) + (r - 1) * Ns + k;
+}
+
 // LDS index with one pad slot per 16 entries (the 16-apart writes of the
 // radix-16 passes would otherwise fall on one bank).
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
@@ -374,10 +390,11 @@ __device__ __forceinline__ void stockham_pass(float2 (&regs)[16], const float2 *
         float2 v[16];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = vin[q * R + r];
-        if (Ns > 1) {
+        if constexpr (Ns > 1) {
+            static_assert(Ns == 16 || Ns == 256, "twiddle layout covers the Ns = 16, 256 passes");
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[(r * k * (M / (Ns * R))) & (M - 1)];
+                float2 w = tw[rho_fast_tw_index(Ns, r, k)];
                 if (INV) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
@@ -413,8 +430,9 @@ void rho_fast_kernel(
     // TAB: the filter H[k] and the gain of every sample, once per workgroup
     float2 *hk = smem + M + FPW * BUF;
     float *gtab = reinterpret_cast<float *>(hk + M);
+    static_assert(rho_fast_tw_count(M) <= M, "twiddles fit the table slot");
     for (int q = tid; q < M; q += kRhoThreads) {
-        tw[q] = twg[q];
+        if (q < rho_fast_tw_count(M)) tw[q] = twg[q];
         if (TAB) {
             hk[q] = rho_response(q, M, wscale, opt);
             gtab[q] = row_gain(q, gain_dt);
@@ -578,12 +596,23 @@ psktm_status twiddles(int dev, int logM, const float2 **out)
     float2 *&slot = g_tw[dev][logM];
     if (!slot) {
         const int M = 1 << logM;
-        // W_M^q = exp(-2 pi i q / M) for q < M (the radix-2 passes read the
-        // first half, the radix-16 passes all of it)
+        // W_M^e = exp(-2 pi i e / M), computed in double
+        auto w = [M](int e) {
+            const double a = -2.0 * M_PI * e / M;
+            return make_float2((float)std::cos(a), (float)std::sin(a));
+        };
+        // the radix-16 kernel's pass-ordered table, else W_M^q for q < M
+        // (the radix-2 passes read the first half)
         std::vector<float2> h(M);
-        for (int q = 0; q < M; ++q) {
-            const double a = -2.0 * M_PI * q / M;
-            h[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+        if (rho_fast_length(M)) {
+            for (int r = 1; r < 16; ++r)
+                for (int k = 0; k < 16; ++k)
+                    h[rho_fast_tw_index(16, r, k)] = w((r * k * (M / 256)) & (M - 1));
+            for (int r = 1; r < M / 256; ++r)
+                for (int k = 0; k < 256; ++k)
+                    h[rho_fast_tw_index(256, r, k)] = w((r * k) & (M - 1));
+        } else {
+            for (int q = 0; q < M; ++q) h[q] = w(q);
         }
         float2 *d = nullptr;
         PSKTM_HIP(hipMalloc(&d, sizeof(float2) * M));
@@ -646,7 +675,7 @@ psktm_status psktm_rho_filter_gain(const float *in, int n_traces, int n_samples,
     if ((st = twiddles(dev, logM, &tw)) != PSKTM_OK) return st;
     const double Dr = (double)dw_us * 1e-6;
     const float wscale = (float)(2.0 * M_PI / Dr);
-    if (M == 1024 || M == 2048 || M == 4096) {
+    if (rho_fast_length(M)) {
         // register-blocked radix-16 form; each workgroup loops over trace
         // pairs (one twiddle-table load and table build per workgroup)
         auto pick = [&](auto mtag) -> const void * {
