@@ -86,11 +86,12 @@ def parse():
                          "bf16, f32 (FP32 plain), f32g (FP32, gained traces); the first is "
                          "the headline")
     ap.add_argument("--workload", default="pstm3d",
-                    choices=["pstm3d", "batched", "lsm", "line2d", "rates"],
+                    choices=["pstm3d", "batched", "lsm", "line2d", "rho", "rates"],
                     help="pstm3d: the headline 3-D workload (default); batched: 64 offset-class "
                          "gathers in one persistent launch; lsm: least-squares 2-D line, 20 CG "
                          "iterations; line2d: the 2-D line, reference-exact GPU form vs the "
-                         "reference CPU loop; rates: gfx950 VALU, v_sqrt_f32, LDS gather and "
+                         "reference CPU loop; rho: the rho filter alone on the headline's "
+                         "traces; rates: gfx950 VALU, v_sqrt_f32, LDS gather and "
                          "LDS atomic rates behind the roofline figures")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache (batched workload)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -151,7 +152,7 @@ def main():
             print(json.dumps({"error": "one GPU only"}))
             sys.exit(2)
         {"batched": bench_batched, "lsm": bench_lsm, "line2d": bench_line2d,
-         "rates": bench_rates}[args.workload](args)
+         "rho": bench_rho, "rates": bench_rates}[args.workload](args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -692,6 +693,83 @@ def bench_line2d(args):
                          "cores": cores, "kind": "port", "host": host_cpu(),
                          "sample": "full 2-D line, best of 3, race-free restatement"},
         "cpu_racy_as_written_G_updates_per_s": updates / racy / 1e9,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_rho(args):
+    """The rho filter alone (scripts/ktmApplyRhoFilter.m, opt 2) on the
+    headline's traces (1e6 x 1,500 samples at 4 ms): FP16 out with the fused
+    sqrt(n dt) gain (what the headline migrates), then FP32 out. Per call:
+    HIP events between back-to-back calls on the launch stream, against the
+    HBM roofline (4 B read and 2 or 4 B written per sample, plus one 4 B
+    scale per trace for FP16). Parity: the first 2,000 traces' FP32 output
+    against the oracle's restatement; CPU baseline: the same restatement
+    (numpy FFTs, one thread) on the first 20,000 traces."""
+    import numpy as np
+    import torch
+
+    import oracle
+    import psktm
+    from psktm import synth
+
+    s = synth.diffractor_3d(args.traces, device="cuda")
+    tr = s.traces
+    n, m = tr.shape
+    dw_us = s.dt * 1e6
+    stream = torch.cuda.current_stream()
+    steps = max(args.steps, 10)
+    forms = {}
+    for form, kw, out_b in (("f16g", dict(out_dtype=torch.float16, gain=True), 2),
+                            ("f32", {}, 4)):
+        for _ in range(max(args.warmup, 2)):
+            psktm.rho_filter(tr, dw_us, 2, **kw)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(stream)
+        for k in range(steps):
+            psktm.rho_filter(tr, dw_us, 2, **kw)
+            ev[k + 1].record(stream)
+        torch.cuda.synchronize()
+        ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(steps)]
+        nbytes = n * m * (4 + out_b) + (4 * n if out_b == 2 else 0)
+        mean = float(np.mean(ms))
+        forms[form] = {"ms_per_call": mean, "best_ms": float(min(ms)), "hbm_bytes": nbytes,
+                       "GB_per_s": nbytes / mean / 1e6, "frac_of_8TB_per_s": nbytes / mean / 8e9}
+        log(f"rho {form}: {mean:.3f} ms per call (best {min(ms):.3f}), "
+            f"{nbytes / mean / 1e6:.0f} GB/s")
+    k = min(n, 2000) // 2 * 2
+    got = psktm.rho_filter(tr[:k].contiguous(), dw_us, 2).cpu().numpy().astype(np.float64)
+    host = tr[:20000].cpu().numpy()
+    ref = oracle.apply_rho_filter(host[:k], dw_us, 2)      # even count: no trace rotation
+    rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    t0 = time.perf_counter()
+    for c in range(0, host.shape[0], 1000):
+        oracle.apply_rho_filter(host[c:c + 1000], dw_us, 2)
+    cpu_s = time.perf_counter() - t0
+    head = forms["f16g"]
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "rho_filter_1M_traces_traffic.json")
+    if os.path.exists(tp):
+        prof = json.load(open(tp))
+        if prof["workload"]["traces"] == n and prof["workload"]["samples"] == m:
+            traffic = prof["hbm_bytes_per_launch"]
+    line = {
+        "metric": "rho filter: trace samples per second (opt 2, FP16 gained out)",
+        "value": n * m / head["ms_per_call"] / 1e6, "unit": "G samples/s", "n_gpus": 1,
+        "steps": steps, "warmup": max(args.warmup, 2), "ms_per_step": head["ms_per_call"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"rho filter, {n} traces x {m} samples (4 ms), 2048-point FFTs"},
+        "roofline": {"bound": "hbm", "achieved": head["GB_per_s"], "peak": 8000.0,
+                     "unit": "GB/s", "frac": head["frac_of_8TB_per_s"],
+                     "traffic": traffic},
+        "forms": forms,
+        "parity": {"traces": k, "rel_l2_fp32_vs_oracle": rel},
+        "cpu_baseline": {"value": host.shape[0] * m / cpu_s / 1e9, "unit": "G samples/s",
+                         "cores": 1, "kind": "port", "host": host_cpu(),
+                         "sample": f"first {host.shape[0]} traces, numpy restatement in "
+                                   "1,000-trace blocks (FP64 FFTs)"},
     }
     print(json.dumps(line), flush=True)
 
