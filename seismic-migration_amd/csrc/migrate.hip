@@ -87,29 +87,36 @@ __global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
 }
 
 // Exact fixed-point accumulation of the adjoint. A lane's merged
-// contribution y, already multiplied by 2^S, is rounded to the integer
-// nearest to it and added as a 64-bit two's complement integer: y splits
-// exactly into hf 2^24 + r (hf = rint(y / 2^24), |r| <= 2^23), and r is
-// rounded by one v_cvt_rpi_i32_f32 (ties up). S is chosen per launch (from
-// adj_max_kernel) so that every merged contribution has |y| < 2^53 and no
-// slot's sum can reach 2^62 in magnitude: the integer sums never overflow,
-// and integer addition is associative, so the adjoint's output does not
-// depend on the order of the LDS atomics. No FP64 is used.
+// contribution y, already multiplied by 2^S, is rounded to the nearest
+// integer (ties to even) and added as a 64-bit two's complement integer.
+// With h = rint(y / 2^22) and r = y - h 2^22 (|r| <= 2^21):
+//   t1 = fma(y, 2^-22, 1.5 2^23) = 1.5 2^23 + h,
+//   t2 = fma(h - 3, -2^22, y)   = 1.5 2^23 + rint(r),
+// each one rounding into [2^23, 2^24), where the floats are the integers
+// (so |y| < 2^43), leaving h and rint(r) in the low mantissa bits. As
+// bits(1.5 2^23) = 301 2^22, the integer is
+//   h 2^22 + rint(r) = (bits(t1) - 301 2^22 - 301) 2^22 + bits(t2),
+// one 64-bit multiply-add. S is chosen per launch (from adj_max_kernel) so
+// that every merged contribution has |y| < 2^43 and no slot's sum can reach
+// 2^62 in magnitude: the integer sums never overflow, and integer addition
+// is associative, so the adjoint's output does not depend on the order of
+// the LDS atomics. No FP64 is used.
 __device__ __forceinline__ unsigned long long to_fixed(float y)
 {
 #pragma clang fp contract(off)
-    const float hf = rintf(y * 5.9604644775390625e-08f);        // rint(y / 2^24), |hf| < 2^29
-    const float r = fmaf(-hf, 16777216.0f, y);                  // y - hf 2^24: exact
+    const float t1 = fmaf(y, 2.384185791015625e-07f, 12582912.0f);
+    const float hm3 = t1 - 12582915.0f;                         // h - 3, exact
+    const float t2 = fmaf(hm3, -4194304.0f, y);
 #if defined(__HIP_DEVICE_COMPILE__)
-    int q;
-    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(q) : "v"(r));
-    // hf 2^24 + q in one 64-bit multiply-add
+    const int hi = (int)(__float_as_uint(t1) - 0x4B40012Du);
     unsigned long long u;
     asm("v_mad_i64_i32 %0, vcc, %1, %2, %3"
-        : "=v"(u) : "v"((int)hf), "s"(16777216), "v"((long long)q) : "vcc");
+        : "=v"(u) : "v"(hi), "s"(4194304), "v"((unsigned long long)__float_as_uint(t2))
+        : "vcc");
     return u;
 #else
-    return (unsigned long long)((long long)(int)hf * 16777216ll + (long long)floorf(r + 0.5f));
+    return (unsigned long long)((long long)(t1 - 12582912.0f) * 4194304ll +
+                                (long long)(t2 - 12582912.0f));
 #endif
 }
 
@@ -162,8 +169,8 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     if (tid < kAdjMax) s_k[tid] = (tid < ng) ? a.sorted_ids[p0 + tid] : -1;
 
     // fixed-point scale 2^S (see to_fixed): mx * cbound < 2^(e1 + e3) bounds
-    // a lane's merged contribution, mx * bound < 2^(e1 + e2) the sum of
-    // |contributions| to a slot
+    // a lane's merged contribution (which must stay below 2^43), mx * bound <
+    // 2^(e1 + e2) the sum of |contributions| to a slot
     const float mx = __uint_as_float(*a.wmax);
     const bool bad = !(mx <= 3.0e38f);           // NaN or inf in the image
     int e1 = 0, e2 = 0, e3 = 0;
@@ -173,7 +180,14 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     // S reaches 250 for images near the FP32 denormal range (and is negative
     // for images near FLT_MAX): applied by ldexp, which rounds once
     const int S = (bad || !(mx > 0.f)) ? 0
-                : min(250, max(-120, min(53 - (e1 + e3), 62 - (e1 + e2))));
+                : min(250, max(-120, min(43 - (e1 + e3), 62 - (e1 + e2))));
+    // 2^S folded into the weight w = tau sqrt(dt) (one instruction fewer per
+    // update) where w 2^S stays finite and well above the subnormals at both
+    // ends of the tau axis: image * (w 2^S) is then the same product as
+    // image * w times 2^S, rounded once
+    const float wend = fmaxf(fabsf(axis_f32(a.tau0, a.dtau, 0)),
+                             fabsf(axis_f32(a.tau0, a.dtau, a.nt - 1))) * a.wscale;
+    const bool fold = S >= -60 && S <= 100 && ldexpf(wend, S) < 1.0e37f;
     __syncthreads();
 
     const int CI = c / a.cells_y, CJ = c % a.cells_y;
@@ -183,7 +197,9 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
     // (src/psktmCPU.c:82)
     const float xhi = a.nsf - 1.f;
     const float xmid = 0.5f * (a.nsf + 1.f), xrad = 0.5f * (a.nsf - 1.f);
-    const int disc = a.ns + 1;
+    // the discard slot in a VGPR (the validity select then reads no SGPR)
+    int disc = a.ns + 1;
+    asm volatile("" : "+v"(disc));
 
     for (int TI = ti0; TI <= ti1; ++TI) {
         for (int TJ = tj0; TJ <= tj1; ++TJ) {
@@ -220,6 +236,7 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                         tq[j] = ok ? a.tsq[l] : 0.f;
                         // weight tau*sqrt(dt) (the forward's epilogue)
                         wt[j] = ok ? (axis_f32(a.tau0, a.dtau, l) * a.wscale) : 0.f;
+                        if (fold) wt[j] = ldexpf(wt[j], S);
                         sl1[j] = (!V3 && ok) ? a.slow[l] : 0.f;
                     }
                     // bins and traces in rolled loops: the kernel's code
@@ -244,9 +261,13 @@ __global__ __launch_bounds__(NTH) void demigrate_kernel(const AdjArgs a)
                                                : vi;
 #pragma unroll
                             for (int e = 0; e < 4; ++e) {
-                                im[q + e] = ldexpf(__uint_as_float(vi[e]) * wt[q + e], S);
+                                im[q + e] = __uint_as_float(vi[e]) * wt[q + e];
                                 sl[q + e] = V3 ? __uint_as_float(vv[e]) : sl1[q + e];
                             }
+                        }
+                        if (!fold) {
+#pragma unroll
+                            for (int j = 0; j < TL; ++j) im[j] = ldexpf(im[j], S);
                         }
 #pragma unroll 1
                         for (int g = 0; g < kAdjMax; ++g) {
