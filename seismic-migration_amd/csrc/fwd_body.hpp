@@ -401,8 +401,11 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             if (!m || s + g >= s1) continue;
             const float2 *P = s_pairs + g * a.nsp + kPairFront;
             const unsigned *PH = reinterpret_cast<const unsigned *>(s_pairs) + g * a.nsp + kPairFront;
-            const int sck = HP ? __builtin_amdgcn_readfirstlane((int)s_mask[kGMax + g]) : 0;
+            // the trace's scale stays in a VGPR (every lane reads the same
+            // LDS word): an accumulate FMA with no SGPR operand can dual-issue
+            float scv = HP ? __uint_as_float(s_mask[kGMax + g]) : 0.f;
 #if defined(__HIP_DEVICE_COMPILE__)
+            asm volatile("" : "+v"(scv));
             // LDS address of this trace's distance terms, moved into a VGPR
             // once per trace by an opaque move: the compiler then reads every
             // bin's (A, B) at an immediate offset from it instead of
@@ -441,7 +444,7 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                         d = fmaf(f, pr.y, pr.x);
                     }
                     if (HP) {
-                        acc[b][j] = fmaf(d, __int_as_float(sck), acc[b][j]);
+                        acc[b][j] = fmaf(d, scv, acc[b][j]);
                     } else if (KH) {
                         // gained traces: no per-update weight
                         float &c = cmp[KH ? b : 0][KH ? j : 0];
