@@ -396,16 +396,23 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             fill_stage<NTH>(a, s, s1, s_pairs);
         __syncthreads();
         // ---- compute --------------------------------------------------------
-        for (int g = 0; g < G; ++g) {
-            const unsigned m = __builtin_amdgcn_readfirstlane(s_mask[g]);
-            if (!m || s + g >= s1) continue;
-            const float2 *P = s_pairs + g * a.nsp + kPairFront;
-            const unsigned *PH = reinterpret_cast<const unsigned *>(s_pairs) + g * a.nsp + kPairFront;
-            // the trace's scale stays in a VGPR (every lane reads the same
-            // LDS word): an accumulate FMA with no SGPR operand can dual-issue
+        // traces staged in this pass (a 32-bit scalar bound: the 64-bit test
+        // s + g < s1 would cost two VALU instructions per trace)
+        const int gn = (int)((s1 - s) < (long long)G ? (s1 - s) : (long long)G);
+        for (int g = 0; g < gn; ++g) {
+            // the trace's scale (read with its mask) stays in a VGPR: every
+            // lane reads the same LDS word, and an accumulate FMA with no
+            // SGPR operand can dual-issue
+            unsigned mv = s_mask[g];
             float scv = HP ? __uint_as_float(s_mask[kGMax + g]) : 0.f;
 #if defined(__HIP_DEVICE_COMPILE__)
-            asm volatile("" : "+v"(scv));
+            asm volatile("" : "+v"(mv), "+v"(scv));
+#endif
+            const unsigned m = __builtin_amdgcn_readfirstlane(mv);
+            if (!m) continue;
+            const float2 *P = s_pairs + g * a.nsp + kPairFront;
+            const unsigned *PH = reinterpret_cast<const unsigned *>(s_pairs) + g * a.nsp + kPairFront;
+#if defined(__HIP_DEVICE_COMPILE__)
             // LDS address of this trace's distance terms, moved into a VGPR
             // once per trace by an opaque move: the compiler then reads every
             // bin's (A, B) at an immediate offset from it instead of
