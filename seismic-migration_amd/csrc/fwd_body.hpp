@@ -158,8 +158,9 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
     }
 }
 
-// Stages rows s .. s+G-1 of FP16 traces as half pairs: the same items,
-// chunks and buffer ranges as fill_stage, each lane writing the 8 entries of
+// Stages rows s .. s+G-1 of FP16 traces as half pairs, one trace per wave at
+// a time, with the chunks and buffer ranges of fill_stage: each lane writes
+// the 8 entries of
 // its chunk as dwords (u[n] | (u[n+1] - u[n]) << 16 with u = h/2, exact for
 // every normal FP16 value, and one FP16 rounding of the difference), zero
 // outside the valid range. Without the halving, neighbours such as +32768 and
@@ -173,83 +174,84 @@ __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, in
     constexpr int W = NTH / 64;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const int groups = ((14 + a.ns * 2) / 16 + 1 + 63) / 64;
     const bool lin = a.interp == PSKTM_INTERP_LINEAR;
     const int lo = lin ? 0 : 1, hi = lin ? a.ns - 2 : a.ns - 1;
-    for (int it = wave; it < G * groups; it += W) {
-        const int g = it / groups, cg = it - g * groups;
+    typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+    const h2v half2 = {(_Float16)0.5f, (_Float16)0.5f};
+    const unsigned lm = lin ? 0xffffu : 0u;     // TRUNC: no difference term
+    // one trace per wave at a time: its row set-up once, then its chunks
+    for (int g = wave; g < G; g += W) {
         const long long q = s + g;
         const int k = a.all_traces ? (int)q : a.list[q];
         const uintptr_t ra = (uintptr_t)a.traces + (size_t)k * a.ns * 2;
         const int sb = (int)(ra & 15u);
         const int n_chunks = (sb + a.ns * 2) / 16 + 1;
-        const int c = cg * 64 + lane;
-        if (c >= n_chunks) continue;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(ra & ~(uintptr_t)15), (short)0, (sb + a.ns * 2 + 3) & ~3, 0x00020000);
-        const int n0 = (16 * c - sb) >> 1;          // rows start on a whole sample
-        unsigned *P = s_half + g * a.nsp + kPairFront + n0;
-        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * c, 0, 0);
-        const unsigned nx = __builtin_amdgcn_raw_buffer_load_b16(rs, 16 * c + 16, 0, 0);
-        // entry i from the sample pair (h[n0+i], h[n0+i+1]) as one dword (a
-        // funnel shift for odd i): both halved by one packed FP16 multiply
-        // with an inline constant, then (u0, u1 - u0)
-        typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-        const h2v half2 = {(_Float16)0.5f, (_Float16)0.5f};
-        const unsigned lm = lin ? 0xffffu : 0u;     // TRUNC: no difference term
-        auto entry = [&](int i) -> unsigned {
-            const unsigned pr = (i & 1) ? __builtin_amdgcn_alignbit(i < 7 ? v[(i + 1) >> 1] : nx,
-                                                                    v[i >> 1], 16)
-                                        : v[i >> 1];
-            const h2v uu = __builtin_bit_cast(h2v, pr) * half2;
-            const unsigned dd = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)(uu.y - uu.x));
-            return (unsigned)__builtin_bit_cast(unsigned short, uu.x) | ((dd & lm) << 16);
-        };
-        // whole waves of interior LINEAR chunks on 16-byte aligned entries
-        // (the alignment is the same for every chunk of a row): no range tests
-        const bool inner = (sb & 6) == 0 && n0 >= lo && n0 + 7 <= hi;
-        if (__builtin_amdgcn_ballot_w64(!inner) == 0 && lin) {
-            // LINEAR: halve the chunk (and the next chunk's first sample) by
-            // packed FP16 multiplies, then make each entry by one packed FMA
-            // of halved samples a = u[n+1], c = u[n]:
-            //   lo = a * 0 + c = u[n],  hi = a * 1 - c = fp16(u[n+1] - u[n])
-            // (the same roundings as entry(): an exact halving, one rounding
-            // of the difference)
-            auto halve = [&](int k) -> unsigned {
-                return __builtin_bit_cast(unsigned, __builtin_bit_cast(h2v, k < 4 ? v[k] : nx) * half2);
+        unsigned *row = s_half + g * a.nsp + kPairFront;
+        // 16-byte aligned entries (the same for every chunk of the row)
+        const bool fast = lin && (sb & 6) == 0;
+        for (int c = lane; c < n_chunks; c += 64) {
+            const int n0 = (16 * c - sb) >> 1;      // rows start on a whole sample
+            unsigned *P = row + n0;
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, 16 * c, 0, 0);
+            const unsigned nx = __builtin_amdgcn_raw_buffer_load_b16(rs, 16 * c + 16, 0, 0);
+            // entry i from the sample pair (h[n0+i], h[n0+i+1]) as one dword
+            // (a funnel shift for odd i): both halved by one packed FP16
+            // multiply with an inline constant, then (u0, u1 - u0)
+            auto entry = [&](int i) -> unsigned {
+                const unsigned pr = (i & 1) ? __builtin_amdgcn_alignbit(i < 7 ? v[(i + 1) >> 1] : nx,
+                                                                        v[i >> 1], 16)
+                                            : v[i >> 1];
+                const h2v uu = __builtin_bit_cast(h2v, pr) * half2;
+                const unsigned dd = (unsigned)__builtin_bit_cast(unsigned short, (_Float16)(uu.y - uu.x));
+                return (unsigned)__builtin_bit_cast(unsigned short, uu.x) | ((dd & lm) << 16);
             };
+            // whole waves of interior chunks: no range tests
+            const bool inner = n0 >= lo && n0 + 7 <= hi;
+            if (fast && __builtin_amdgcn_ballot_w64(!inner) == 0) {
+                // LINEAR: halve the chunk (and the next chunk's first sample)
+                // by packed FP16 multiplies, then make each entry by one
+                // packed FMA of halved samples a = u[n+1], c = u[n]:
+                //   lo = a * 0 + c = u[n],  hi = a * 1 - c = fp16(u[n+1] - u[n])
+                // (the same roundings as entry(): an exact halving, one
+                // rounding of the difference)
+                auto halve = [&](int k) -> unsigned {
+                    return __builtin_bit_cast(unsigned, __builtin_bit_cast(h2v, k < 4 ? v[k] : nx) * half2);
+                };
 #if defined(__HIP_DEVICE_COMPILE__)
-            const unsigned zero_one = 0x3c000000u;          // (0.0, 1.0) as FP16 (lo, hi)
-            // even entry: (u.lo, u.hi - u.lo)
-            auto even = [&](unsigned u) -> unsigned {
-                unsigned e;
-                asm("v_pk_fma_f16 %0, %1, %2, %1 op_sel_hi:[1,1,0] neg_hi:[0,0,1]"
-                    : "=v"(e) : "v"(u), "s"(zero_one));
-                return e;
-            };
-            // odd entry: (u.hi, w.lo - u.hi), w the next dword
-            auto odd = [&](unsigned u, unsigned w) -> unsigned {
-                unsigned e;
-                asm("v_pk_fma_f16 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,1,1] neg_hi:[0,0,1]"
-                    : "=v"(e) : "v"(w), "s"(zero_one), "v"(u));
-                return e;
-            };
-            // the first four entries are stored before the rest are made
-            const unsigned u0 = halve(0), u1 = halve(1), u2 = halve(2);
-            reinterpret_cast<uint4 *>(P)[0] = make_uint4(even(u0), odd(u0, u1), even(u1), odd(u1, u2));
-            const unsigned u3 = halve(3), u4 = halve(4);
-            reinterpret_cast<uint4 *>(P)[1] = make_uint4(even(u2), odd(u2, u3), even(u3), odd(u3, u4));
+                const unsigned zero_one = 0x3c000000u;      // (0.0, 1.0) as FP16 (lo, hi)
+                // even entry: (u.lo, u.hi - u.lo)
+                auto even = [&](unsigned u) -> unsigned {
+                    unsigned r;
+                    asm("v_pk_fma_f16 %0, %1, %2, %1 op_sel_hi:[1,1,0] neg_hi:[0,0,1]"
+                        : "=v"(r) : "v"(u), "s"(zero_one));
+                    return r;
+                };
+                // odd entry: (u.hi, w.lo - u.hi), w the next dword
+                auto odd = [&](unsigned u, unsigned w) -> unsigned {
+                    unsigned r;
+                    asm("v_pk_fma_f16 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,1,1] neg_hi:[0,0,1]"
+                        : "=v"(r) : "v"(w), "s"(zero_one), "v"(u));
+                    return r;
+                };
+                // the first four entries are stored before the rest are made
+                const unsigned u0 = halve(0), u1 = halve(1), u2 = halve(2);
+                reinterpret_cast<uint4 *>(P)[0] = make_uint4(even(u0), odd(u0, u1), even(u1), odd(u1, u2));
+                const unsigned u3 = halve(3), u4 = halve(4);
+                reinterpret_cast<uint4 *>(P)[1] = make_uint4(even(u2), odd(u2, u3), even(u3), odd(u3, u4));
 #else
-            (void)halve;
-            reinterpret_cast<uint4 *>(P)[0] = make_uint4(entry(0), entry(1), entry(2), entry(3));
-            reinterpret_cast<uint4 *>(P)[1] = make_uint4(entry(4), entry(5), entry(6), entry(7));
+                (void)halve;
+                reinterpret_cast<uint4 *>(P)[0] = make_uint4(entry(0), entry(1), entry(2), entry(3));
+                reinterpret_cast<uint4 *>(P)[1] = make_uint4(entry(4), entry(5), entry(6), entry(7));
 #endif
-        } else {
-            // row edges, unaligned rows and TRUNC: entry by entry
+            } else {
+                // row edges, unaligned rows and TRUNC: entry by entry
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const bool ok = (unsigned)(n0 + i - lo) <= (unsigned)(hi - lo);
-                P[i] = ok ? entry(i) : 0u;
+                for (int i = 0; i < 8; ++i) {
+                    const bool ok = (unsigned)(n0 + i - lo) <= (unsigned)(hi - lo);
+                    P[i] = ok ? entry(i) : 0u;
+                }
             }
         }
     }
