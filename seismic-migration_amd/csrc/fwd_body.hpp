@@ -160,10 +160,9 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
 
 // Stages rows s .. s+G-1 of FP16 traces as half pairs, one trace per wave at
 // a time, with the chunks and buffer ranges of fill_stage: each lane writes
-// the 8 entries of
-// its chunk as dwords (u[n] | (u[n+1] - u[n]) << 16 with u = h/2, exact for
-// every normal FP16 value, and one FP16 rounding of the difference), zero
-// outside the valid range. Without the halving, neighbours such as +32768 and
+// the 8 entries of its chunk as dwords (u[n] | (u[n+1] - u[n]) << 16 with
+// u = h/2, exact for every normal FP16 value, and one FP16 rounding of the
+// difference), zero outside the valid range. Without the halving, neighbours such as +32768 and
 // -32752 (both valid quantised samples) would difference to -inf. Rows start
 // 16-byte aligned, so a chunk whose first entry is a multiple of 4 writes two
 // 16-byte stores.
