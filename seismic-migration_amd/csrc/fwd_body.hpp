@@ -208,44 +208,67 @@ __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, in
             };
             // whole waves of interior chunks: no range tests
             const bool inner = n0 >= lo && n0 + 7 <= hi;
-            if (fast && __builtin_amdgcn_ballot_w64(!inner) == 0) {
-                // LINEAR: halve the chunk (and the next chunk's first sample)
-                // by packed FP16 multiplies, then make each entry by one
-                // packed FMA of halved samples a = u[n+1], c = u[n]:
-                //   lo = a * 0 + c = u[n],  hi = a * 1 - c = fp16(u[n+1] - u[n])
-                // (the same roundings as entry(): an exact halving, one
-                // rounding of the difference)
-                auto halve = [&](int k) -> unsigned {
-                    return __builtin_bit_cast(unsigned, __builtin_bit_cast(h2v, k < 4 ? v[k] : nx) * half2);
-                };
+            // LINEAR on aligned rows: halve the chunk (and the next chunk's
+            // first sample) by packed FP16 multiplies, then make each entry by
+            // one packed FMA of halved samples a = u[n+1], c = u[n]:
+            //   lo = a * 0 + c = u[n],  hi = a * 1 - c = fp16(u[n+1] - u[n])
+            // (the same roundings as entry(): an exact halving, one rounding
+            // of the difference)
+            auto halve = [&](int k) -> unsigned {
+                return __builtin_bit_cast(unsigned, __builtin_bit_cast(h2v, k < 4 ? v[k] : nx) * half2);
+            };
 #if defined(__HIP_DEVICE_COMPILE__)
-                const unsigned zero_one = 0x3c000000u;      // (0.0, 1.0) as FP16 (lo, hi)
-                // even entry: (u.lo, u.hi - u.lo)
-                auto even = [&](unsigned u) -> unsigned {
-                    unsigned r;
-                    asm("v_pk_fma_f16 %0, %1, %2, %1 op_sel_hi:[1,1,0] neg_hi:[0,0,1]"
-                        : "=v"(r) : "v"(u), "s"(zero_one));
-                    return r;
-                };
-                // odd entry: (u.hi, w.lo - u.hi), w the next dword
-                auto odd = [&](unsigned u, unsigned w) -> unsigned {
-                    unsigned r;
-                    asm("v_pk_fma_f16 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,1,1] neg_hi:[0,0,1]"
-                        : "=v"(r) : "v"(w), "s"(zero_one), "v"(u));
-                    return r;
-                };
+            const unsigned zero_one = 0x3c000000u;          // (0.0, 1.0) as FP16 (lo, hi)
+            // even entry: (u.lo, u.hi - u.lo)
+            auto even = [&](unsigned u) -> unsigned {
+                unsigned r;
+                asm("v_pk_fma_f16 %0, %1, %2, %1 op_sel_hi:[1,1,0] neg_hi:[0,0,1]"
+                    : "=v"(r) : "v"(u), "s"(zero_one));
+                return r;
+            };
+            // odd entry: (u.hi, w.lo - u.hi), w the next dword
+            auto odd = [&](unsigned u, unsigned w) -> unsigned {
+                unsigned r;
+                asm("v_pk_fma_f16 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,1,1] neg_hi:[0,0,1]"
+                    : "=v"(r) : "v"(w), "s"(zero_one), "v"(u));
+                return r;
+            };
+#else
+            auto even = [&](unsigned) -> unsigned { return 0u; };
+            auto odd = [&](unsigned, unsigned) -> unsigned { return 0u; };
+#endif
+            if (fast && __builtin_amdgcn_ballot_w64(!inner) == 0) {
+#if defined(__HIP_DEVICE_COMPILE__)
                 // the first four entries are stored before the rest are made
                 const unsigned u0 = halve(0), u1 = halve(1), u2 = halve(2);
                 reinterpret_cast<uint4 *>(P)[0] = make_uint4(even(u0), odd(u0, u1), even(u1), odd(u1, u2));
                 const unsigned u3 = halve(3), u4 = halve(4);
                 reinterpret_cast<uint4 *>(P)[1] = make_uint4(even(u2), odd(u2, u3), even(u3), odd(u3, u4));
 #else
-                (void)halve;
                 reinterpret_cast<uint4 *>(P)[0] = make_uint4(entry(0), entry(1), entry(2), entry(3));
                 reinterpret_cast<uint4 *>(P)[1] = make_uint4(entry(4), entry(5), entry(6), entry(7));
 #endif
+            } else if (fast) {
+                // waves holding a row end: the same entries one at a time,
+                // zero outside [lo, hi]
+                auto put = [&](int i, unsigned e) {
+                    P[i] = (unsigned)(n0 + i - lo) <= (unsigned)(hi - lo) ? e : 0u;
+                };
+#if defined(__HIP_DEVICE_COMPILE__)
+                unsigned u = halve(0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const unsigned w = halve(k + 1);
+                    put(2 * k, even(u));
+                    put(2 * k + 1, odd(u, w));
+                    u = w;
+                }
+#else
+#pragma unroll
+                for (int i = 0; i < 8; ++i) put(i, entry(i));
+#endif
             } else {
-                // row edges, unaligned rows and TRUNC: entry by entry
+                // unaligned rows and TRUNC: entry by entry
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const bool ok = (unsigned)(n0 + i - lo) <= (unsigned)(hi - lo);

@@ -73,8 +73,17 @@ struct AdjArgs {
     float *traces;
 };
 
+// Resident waves per SIMD asked of the forward kernel: three 512-lane
+// workgroups per CU (at most 80 VGPRs) for the plain 3-D tiles of three tau
+// samples per lane; otherwise whatever the registers allow.
+constexpr int fwd_waves_per_eu(int nth, int tl, bool kh)
+{
+    return nth == 512 && tl <= 3 && !kh ? 6 : 1;
+}
+
 template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP>
-__global__ __launch_bounds__(NTH) void migrate_fwd_kernel(const FwdArgs a)
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(fwd_waves_per_eu(NTH, TL, KH))))
+void migrate_fwd_kernel(const FwdArgs a)
 {
     extern __shared__ __align__(16) float2 s_pairs[];
     __shared__ float2 s_ab[kGMax][TB];
