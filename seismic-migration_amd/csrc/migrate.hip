@@ -74,15 +74,17 @@ struct AdjArgs {
 };
 
 // Resident waves per SIMD asked of the forward kernel: three 512-lane
-// workgroups per CU (at most 80 VGPRs) for the plain 3-D tiles of three tau
-// samples per lane; otherwise whatever the registers allow.
-constexpr int fwd_waves_per_eu(int nth, int tl, bool kh)
+// workgroups per CU (at most 80 VGPRs) for the half-pair 3-D tiles of three
+// tau samples per lane, whose fill alone would take it past 80; the other
+// forms fit as compiled (asking them too changes their schedule: FP32 pairs
+// ran 5% slower).
+constexpr int fwd_waves_per_eu(int nth, int tl, bool hp)
 {
-    return nth == 512 && tl <= 3 && !kh ? 6 : 1;
+    return nth == 512 && tl <= 3 && hp ? 6 : 1;
 }
 
 template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(fwd_waves_per_eu(NTH, TL, KH))))
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(fwd_waves_per_eu(NTH, TL, HP))))
 void migrate_fwd_kernel(const FwdArgs a)
 {
     extern __shared__ __align__(16) float2 s_pairs[];
