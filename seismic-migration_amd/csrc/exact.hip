@@ -18,7 +18,17 @@
 // terms A = |s-x|^2, B = |r-x|^2 (src/psktmCPU.c:65-72) in LDS; trace samples
 // are gathered from global memory (neighbouring lanes read neighbouring
 // samples of one trace, so the gather stays within a few cache lines).
+//
+// Images too small to fill the GPU that way (a few hundred workgroups or
+// fewer, e.g. a 64-bin line) take the split form: workgroups (x, y, z) compute
+// the contributions of trace stage z into a scratch array, without summing,
+// and a second kernel adds them to each image sample one trace at a time in
+// increasing k order. The sums are the direct form's: a skipped contribution
+// is added as +0.0, which leaves every sum unchanged (sums start at +0.0 and
+// never become -0.0 under round-to-nearest).
 #include "internal.hpp"
+
+#include <algorithm>
 
 #pragma clang fp contract(off)
 
@@ -37,12 +47,18 @@ __device__ __forceinline__ float axis_value(float start, float step, int i)
     return start + p;
 }
 
+// TERMS: the split form's first kernel. Workgroup z takes traces
+// kbase + z kStage .. (at most kStage, below kend) and writes each
+// contribution (+0.0 where the reference skips the sample) to
+// terms[(k - kbase) * nbins * nt + b * nt + l].
+template <bool TERMS>
 __global__ __launch_bounds__(kThreads) void pstm_exact_kernel(
     int nx, float x0, float dx, int ny, float y0, float dy, int nt, float tau0,
     float dtau, int ntr, int ns, float dt, const float *__restrict__ midX,
     const float *__restrict__ midY, const float *__restrict__ offX,
     const float *__restrict__ offY, const float *__restrict__ traces,
-    const float *__restrict__ vel, float *__restrict__ image)
+    const float *__restrict__ vel, float *__restrict__ image, int kbase, int kend,
+    float *__restrict__ terms)
 {
     __shared__ float sA[kStage][kBins];
     __shared__ float sB[kStage][kBins];
@@ -66,8 +82,11 @@ __global__ __launch_bounds__(kThreads) void pstm_exact_kernel(
 #pragma unroll
     for (int bb = 0; bb < kBins; ++bb) acc[bb] = 0.0f;
 
-    for (int k0 = 0; k0 < ntr; k0 += kStage) {
-        const int kn = min(kStage, ntr - k0);
+    const int kfirst = TERMS ? kbase + (int)blockIdx.z * kStage : 0;
+    const int klast = TERMS ? min(kend, kfirst + kStage) : ntr;
+    const size_t nsamp = (size_t)nbins * (size_t)nt;
+    for (int k0 = kfirst; k0 < klast; k0 += kStage) {
+        const int kn = min(kStage, klast - k0);
         for (int q = threadIdx.x; q < kStage * kBins; q += kThreads) {
             const int g = q / kBins, bb = q % kBins;
             const long b = b0 + bb;
@@ -101,23 +120,49 @@ __global__ __launch_bounds__(kThreads) void pstm_exact_kernel(
                     const float q = t / dt;                        // :80
                     // (int)q lies in (0, ns) exactly when 1 <= q < ns
                     // (truncation of a non-negative value; NaN fails both).
+                    float c = 0.0f;
                     if (q >= 1.0f && q < nsf) {                    // :82
                         const int n = (int)q;
                         const float dtm = tr[n] * t;               // :84-85
                         const float r = tauS / t;
-                        acc[bb] = acc[bb] + dtm * sqrtf(r);
+                        c = dtm * sqrtf(r);
+                        if (!TERMS) acc[bb] = acc[bb] + c;
                     }
+                    if (TERMS && b0 + bb < nbins)
+                        terms[(size_t)(k0 + g - kbase) * nsamp + (size_t)(b0 + bb) * nt + l] = c;
                 }
             }
         }
         __syncthreads();
     }
-    if (l < nt) {
+    if (!TERMS && l < nt) {
 #pragma unroll
         for (int bb = 0; bb < kBins; ++bb)
             if (b0 + bb < nbins) image[(size_t)(b0 + bb) * (size_t)nt + l] = acc[bb];
     }
 }
+
+// The split form's second kernel: image[s] += terms[k][s] for k = 0 .. kn-1
+// in order (one binary32 add each), one lane per image sample.
+__global__ __launch_bounds__(kThreads) void pstm_exact_sum_kernel(
+    size_t nsamp, int kn, const float *__restrict__ terms, float *__restrict__ image)
+{
+    const size_t s = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (s >= nsamp) return;
+    float acc = image[s];
+#pragma unroll 8
+    for (int k = 0; k < kn; ++k) acc = acc + terms[(size_t)k * nsamp + s];
+    image[s] = acc;
+}
+
+__global__ void zero_image_kernel(float *__restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// Scratch of the split form: at most this many bytes of contributions.
+constexpr size_t kTermsBytes = size_t(1) << 30;
 
 } // namespace
 
@@ -131,11 +176,47 @@ psktm_status launch_exact(const psktm_grid &g, const psktm_survey &s,
     PSKTM_CHECK(bx <= 2147483647L && ty <= 65535, PSKTM_E_INVAL,
                 "image too large for one launch (%ld bins, %d samples)", nbins, g.nt);
     dim3 grid((unsigned)bx, (unsigned)ty);
-    hipLaunchKernelGGL(pstm_exact_kernel, grid, dim3(kThreads), 0, stream, g.nx,
-                       g.x0, g.dx, g.ny, g.y0, g.dy, g.nt, g.tau0, g.dtau,
-                       s.n_traces, s.n_samples, s.dt, s.mid_x, s.mid_y, s.off_x,
-                       s.off_y, traces, vel, image);
-    return check_launch("pstm_exact_kernel");
+    auto direct = [&]() {
+        hipLaunchKernelGGL(pstm_exact_kernel<false>, grid, dim3(kThreads), 0, stream, g.nx,
+                           g.x0, g.dx, g.ny, g.y0, g.dy, g.nt, g.tau0, g.dtau,
+                           s.n_traces, s.n_samples, s.dt, s.mid_x, s.mid_y, s.off_x,
+                           s.off_y, traces, vel, image, 0, 0, (float *)nullptr);
+        return check_launch("pstm_exact_kernel");
+    };
+    // the split form where the direct grid leaves most CUs idle and at least
+    // one stage of contributions fits the scratch budget
+    int dev = 0, cus = 0;
+    PSKTM_HIP(hipGetDevice(&dev));
+    PSKTM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t nsamp = (size_t)nbins * (size_t)g.nt;
+    const size_t per_trace = nsamp * sizeof(float);
+    if (bx * ty >= 2L * cus || s.n_traces < 2 * kStage || per_trace * kStage > kTermsBytes)
+        return direct();
+    // traces per pass: as many whole stages as the budget holds (grid z <= 65535)
+    long ks = (long)std::min<size_t>(kTermsBytes / per_trace, (size_t)65535 * kStage);
+    ks = std::min<long>(ks / kStage * kStage, s.n_traces);
+    float *terms = nullptr;
+    if (hipMallocAsync((void **)&terms, (size_t)ks * per_trace, stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return direct();                      // no scratch: the direct form
+    }
+    hipLaunchKernelGGL(zero_image_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0,
+                       stream, image, nsamp);
+    psktm_status st = check_launch("zero_image_kernel");
+    for (long k0 = 0; st == PSKTM_OK && k0 < s.n_traces; k0 += ks) {
+        const int kn = (int)std::min<long>(ks, s.n_traces - k0);
+        const dim3 gz((unsigned)bx, (unsigned)ty, (unsigned)((kn + kStage - 1) / kStage));
+        hipLaunchKernelGGL(pstm_exact_kernel<true>, gz, dim3(kThreads), 0, stream, g.nx,
+                           g.x0, g.dx, g.ny, g.y0, g.dy, g.nt, g.tau0, g.dtau,
+                           s.n_traces, s.n_samples, s.dt, s.mid_x, s.mid_y, s.off_x,
+                           s.off_y, traces, vel, image, (int)k0, (int)k0 + kn, terms);
+        if ((st = check_launch("pstm_exact_kernel")) != PSKTM_OK) break;
+        hipLaunchKernelGGL(pstm_exact_sum_kernel, dim3((unsigned)((nsamp + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, stream, nsamp, kn, (const float *)terms, image);
+        st = check_launch("pstm_exact_sum_kernel");
+    }
+    (void)hipFreeAsync(terms, stream);
+    return st;
 }
 
 } // namespace psktm

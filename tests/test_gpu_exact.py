@@ -58,9 +58,14 @@ def test_legacy_ktm_migration_bit_exact(cuda, entry):
 @pytest.mark.parametrize("seed,ntr,nx,ny,nt,ns", [
     (0, 37, 5, 3, 300, 260),        # ragged: bins not a multiple of 8, nt not of 256
     (1, 1, 1, 1, 1, 2),             # single trace, single bin, single sample
-    (2, 70, 9, 7, 513, 400),        # 3-D, nonzero offY, many stages
+    (2, 70, 9, 7, 513, 400),        # 3-D, nonzero offY, split form: 2 stages + a partial one
+    (3, 100, 64, 64, 128, 300),     # direct form with many stages (512 workgroups)
+    (4, 600, 30, 30, 1000, 1200),   # split form in three passes (288 + 288 + 24 traces)
 ])
 def test_random_3d_bit_exact(cuda, seed, ntr, nx, ny, nt, ns):
+    """Images under a few hundred workgroups of the direct form take the
+    split form (contributions in parallel, then summed per sample in k
+    order); both must equal the single-thread loop bit for bit."""
     rng = np.random.default_rng(seed)
     grid = (nx, -30.0, 17.5, ny, 12.0, 22.5, nt, 0.0, 0.003)
     mid = rng.uniform(-100, 300, (ntr, 2)).astype(np.float32)
