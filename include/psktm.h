@@ -304,7 +304,12 @@ psktm_status psktm_batch_migrate(psktm_batch *batch, const psktm_traces *traces,
                                  int use_graph, void *stream);
 
 /* Reference-exact migration on device pointers (the kernel behind
- * ktmMigrationGPU): TRUNC + IEEE, 1-D velocity v[nt], no aperture, FP32. */
+ * ktmMigrationGPU): TRUNC + IEEE, 1-D velocity v[nt], no aperture, FP32.
+ * Images too small to fill the GPU with one lane per sample (under two
+ * workgroups of 8 bins x 256 tau per CU) are computed in trace stages into
+ * stream-ordered scratch (hipMallocAsync/hipFreeAsync on `stream`, at most
+ * 1 GiB) and summed per sample in trace order; the image bits are the same.
+ * Without that memory the call falls back to the one-lane-per-sample form. */
 psktm_status psktm_migrate_exact(const psktm_grid *grid,
                                  const psktm_survey *survey,
                                  const float *traces, const float *vel,
