@@ -409,8 +409,9 @@ __device__ __forceinline__ void stockham_pass(float2 (&regs)[16], const float2 *
 }
 
 // One workgroup of 256 lanes runs 256/L transforms at a time (two traces
-// each) and loops over trace groups; the twiddle table W_M^e (e < M) and one
-// padded exchange buffer per transform live in LDS.
+// each) and loops over trace groups; the pass-ordered twiddle table, the
+// filter and gain tables (M <= 2048) and one padded exchange buffer per
+// transform live in LDS.
 template <int M, int DT>
 __global__ __launch_bounds__(kRhoThreads) __attribute__((amdgpu_waves_per_eu(kRhoWaves)))
 void rho_fast_kernel(
