@@ -90,14 +90,16 @@ def test_zero_traces_gives_zero_image(cuda):
     assert not torch.any(img)
 
 
-def test_degenerate_velocity_matches_reference(cuda):
-    """v = 0 makes 1/v^2 infinite: inf and NaN traveltimes are skipped by both."""
+@pytest.mark.parametrize("ntr", [8, 96])      # the direct form, then the split form
+def test_degenerate_velocity_matches_reference(cuda, ntr):
+    """v = 0 makes 1/v^2 infinite: inf and NaN traveltimes are skipped by both
+    (the split form adds +0.0 for them)."""
     rng = np.random.default_rng(9)
     grid = (4, 0.0, 20.0, 2, 0.0, 20.0, 64, 0.0, 0.004)
-    mid = rng.uniform(0, 60, (8, 2)).astype(np.float32)
+    mid = rng.uniform(0, 60, (ntr, 2)).astype(np.float32)
     mid[0] = [20.0, 0.0]                      # a trace exactly on a bin: A = 0 -> 0*inf
-    off = np.zeros((8, 2), np.float32)
-    tr = rng.standard_normal((8, 100)).astype(np.float32)
+    off = np.zeros((ntr, 2), np.float32)
+    tr = rng.standard_normal((ntr, 100)).astype(np.float32)
     vel = rng.uniform(1500, 2500, 64).astype(np.float32)
     vel[10:14] = 0.0
     gpu = _exact_gpu(cuda, grid, mid[:, 0], mid[:, 1], off[:, 0], off[:, 1], tr, vel, 0.004)
