@@ -448,6 +448,51 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                          : "s"((unsigned)(uintptr_t)(const void *)&s_ab[g][0]));
             typedef __attribute__((address_space(3))) const float2 lds_float2;
 #endif
+            if constexpr (KH) {
+                // Precise mode with every bin live in this trace and the next
+                // (a plan without an aperture cut): both traces in one block
+                // with no branch per bin, each Kahan accumulator updated twice
+                // in place. After one update in a block that a skipped bin
+                // would bypass, the sum must be moved back into its register
+                // (one move per update). Same operations, same order.
+                constexpr unsigned full = TB >= 32 ? 0xffffffffu : (1u << TB) - 1u;
+                if (m == full && g + 1 < gn &&
+                    __builtin_amdgcn_readfirstlane(s_mask[g + 1]) == full) {
+                    auto kahan = [&](float2 ab, const float2 *Pt, int b, int j) {
+                        const float sl = sv[V3 ? b : 0][j];
+                        const float ua = fmaf(ab.x, sl, tq[j]);
+                        const float ub = fmaf(ab.y, sl, tq[j]);
+                        float x = __builtin_amdgcn_sqrtf(ua) + __builtin_amdgcn_sqrtf(ub);
+                        if (CL) x = fminf(x, a.nsf);
+                        const int n = (int)x;
+                        const float f = __builtin_amdgcn_fractf(x);
+                        const float2 pr = Pt[n];
+                        const float d = fmaf(f, pr.y, pr.x);
+                        float &c = cmp[KH ? b : 0][KH ? j : 0];
+                        const float y = GN ? d - c : fmaf(d, __builtin_amdgcn_sqrtf(x), -c);
+                        const float t = acc[b][j] + y;
+                        c = (t - acc[b][j]) - y;
+                        acc[b][j] = t;
+                    };
+                    const float2 *P2 = P + a.nsp;
+#pragma unroll
+                    for (int b = 0; b < TB; ++b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                        const float2 ab0 = *(lds_float2 *)(uintptr_t)(abase + 8u * b);
+                        const float2 ab1 = *(lds_float2 *)(uintptr_t)(abase + 8u * (TB + b));
+#else
+                        const float2 ab0 = s_ab[g][b], ab1 = s_ab[g + 1][b];
+#endif
+#pragma unroll
+                        for (int j = 0; j < TL; ++j) {
+                            kahan(ab0, P, b, j);
+                            kahan(ab1, P2, b, j);
+                        }
+                    }
+                    ++g;
+                    continue;
+                }
+            }
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
                 if (!((m >> b) & 1u)) continue;
