@@ -213,7 +213,8 @@ typedef struct psktm_options {
     int   precise;            /* 1: compensated (Kahan) FP32 accumulation in
                                  the forward, for least squares and adjoint
                                  tests. 0: plain FP32 accumulation. The
-                                 adjoint accumulates in FP64 either way. */
+                                 adjoint's sums are exact 64-bit fixed-point
+                                 integers either way. */
     int   splits;             /* forward: workgroups sharing one tile's trace
                                  list, each summing a contiguous part into a
                                  partial image that a second kernel adds in
