@@ -532,9 +532,13 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
     const size_t lds = L.lds;
     { psktm_status s_; if ((s_ = ensure_dynamic_lds((const void *)fn, lds)) != PSKTM_OK) return s_; }
 
-    // Split-K: when the image has too few (tile, tau pass) work units to fill
-    // the GPU twice over, several workgroups share each tile's trace list and
-    // write partial images that reduce_partials_kernel adds in split order.
+    // Split-K: several workgroups share each tile's trace list and write
+    // partial images that reduce_partials_kernel adds in split order. When
+    // the image has too few (tile, tau pass) work units to fill the GPU twice
+    // over, enough splits to do so; when it has fewer than ~40 rounds of
+    // resident workgroups, up to 8 splits so that the last round (the
+    // smallest tiles, launched last) is short: the 1e6-trace 3-D workload
+    // (5,000 tiles, 768 resident workgroups) ran 2.6% faster with 6-7 splits.
     const long long base = (long long)p->n_tiles * sh.chunks;
     int n_split = p->opt.splits;
     if (n_split == 0) {
@@ -542,14 +546,17 @@ psktm_status launch_migrate(psktm_plan *p, const psktm_traces &tr, const psktm_v
         PSKTM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
         PSKTM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn,
                                                                sh.nth, lds));
-        const long long target = 2ll * cus * std::max(per_cu, 1);
+        const long long slots = (long long)cus * std::max(per_cu, 1);
         const double per_tile = p->all_traces ? (double)p->n_traces
                                               : (double)p->list_len / std::max(p->n_tiles, 1);
+        const long long by_len = (long long)(per_tile / (4.0 * G));   // >= 4 stages each
         n_split = 1;
-        if (base > 0 && base < target) {
-            const long long want = (target + base - 1) / base;
-            const long long by_len = (long long)(per_tile / (4.0 * G));   // >= 4 stages each
+        if (base > 0 && base < 2 * slots) {
+            const long long want = (2 * slots + base - 1) / base;
             n_split = (int)std::max(1ll, std::min({want, by_len, 64ll}));
+        } else if (base > 0 && base < 40 * slots) {
+            const long long want = (40 * slots + base - 1) / base;
+            n_split = (int)std::max(1ll, std::min({want, by_len, 8ll}));
         }
     }
     a.n_split = n_split;
