@@ -22,6 +22,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -30,6 +31,7 @@ namespace psktm {
 namespace {
 
 constexpr int kListThreads = 256;
+constexpr int kListU = 4;   // candidates per lane per pass of the list kernels
 
 struct Lattice {
     int nx, ny;
@@ -131,107 +133,178 @@ __device__ __forceinline__ double point_rect_d2(double px, double py, double x0,
     return ex * ex + ey * ey;
 }
 
-// Ordered compaction of one predicate per thread: returns the rank of this
-// thread among the set predicates of lower thread index; *total = count.
-__device__ __forceinline__ int block_rank(bool pred, int *s_wave, int *total)
+// Conservative cell test: the rectangle of midpoints that cell (ci, cj) can
+// hold (midpoints of cell c along x lie in [x0 + ((c-m)*tile - 1/2) dx,
+// x0 + ((c-m+1)*tile - 1/2) dx)) comes within the aperture of the tile's bin
+// rectangle [rx0, rx1] x [ry0, ry1]. Along cj at fixed ci the distance is
+// convex, so the passing cells of a column form one run of cells.
+__device__ __forceinline__ bool cell_in_reach(const Lattice &L, int ci, int cj, double rx0,
+                                              double rx1, double ry0, double ry1)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nwaves = blockDim.x >> 6;
-    unsigned long long bal = __ballot(pred);
-    unsigned long long below = lane ? (bal & ((1ull << lane) - 1ull)) : 0ull;
-    int r = __popcll(below);
-    __syncthreads();
-    if (lane == 0) s_wave[wave] = __popcll(bal);
-    __syncthreads();
-    int base = 0, tot = 0;
-    for (int w = 0; w < nwaves; ++w) {
-        int c = s_wave[w];
-        if (w < wave) base += c;
-        tot += c;
+    double ex = 0.0, ey = 0.0;
+    if (L.cells_x > 1) {
+        const double cx0 = L.x0 + ((double)(ci - L.cell_mx) * L.tile_x - 0.5) * L.dx;
+        const double cx1 = cx0 + (double)L.tile_x * L.dx;
+        ex = fmax(fmax(cx0 - rx1, rx0 - cx1), 0.0);
     }
-    *total = tot;
-    return base + r;
+    if (L.cells_y > 1) {
+        const double cy0 = L.y0 + ((double)(cj - L.cell_my) * L.tile_y - 0.5) * L.dy;
+        const double cy1 = cy0 + (double)L.tile_y * L.dy;
+        ey = fmax(fmax(cy0 - ry1, ry0 - cy1), 0.0);
+    }
+    return !(ex * ex + ey * ey > L.rm2);
 }
 
 // One workgroup per output tile. COUNT: number of listed traces and of
 // (trace, bin) pairs passing the binary32 rule. FILL: writes the list.
+//
+// The tile walks the cell columns within reach; in each column the cells
+// that pass the cell test are one run of sorted positions, read as one
+// stream of midpoints (smid: sorted position -> (mid x, mid y)), kListU
+// candidates per lane per pass. The list keeps sorted-position order (cell
+// order, then trace id): FILL ranks each pass's survivors by (slice, lane)
+// with one exchange of wave counts; COUNT only needs totals and runs with no
+// barrier per pass. Should rounding ever break a column's run, its passing
+// cells are read one by one (same list).
 template <bool FILL>
 __global__ __launch_bounds__(kListThreads) void tile_list_kernel(
-    Lattice L, const float4 *__restrict__ geo, const int *__restrict__ sorted_ids,
+    Lattice L, const float2 *__restrict__ smid, const int *__restrict__ sorted_ids,
     const int *__restrict__ cell_start, long long *__restrict__ tile_count,
     unsigned long long *__restrict__ tile_pairs,
     const long long *__restrict__ tile_off, int *__restrict__ list)
 {
-    __shared__ int s_wave[kListThreads / 64];
+    constexpr int kWaves = kListThreads / 64;
+    __shared__ int s_cnt[kListU * kWaves];
+    __shared__ int s_lo, s_hi, s_n;
     __shared__ unsigned long long s_pairs;
+    __shared__ long long s_count;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int T = blockIdx.x;
     const int TI = T / L.tiles_y, TJ = T % L.tiles_y;
     const int i_lo = TI * L.tile_x, i_hi = min(L.nx, i_lo + L.tile_x) - 1;
     const int j_lo = TJ * L.tile_y, j_hi = min(L.ny, j_lo + L.tile_y) - 1;
     const double rx0 = axis_d(L.x0, L.dx, i_lo), rx1 = axis_d(L.x0, L.dx, i_hi);
     const double ry0 = axis_d(L.y0, L.dy, j_lo), ry1 = axis_d(L.y0, L.dy, j_hi);
-    if (!FILL && threadIdx.x == 0) s_pairs = 0ull;
+    if (!FILL && tid == 0) {
+        s_pairs = 0ull;
+        s_count = 0;
+    }
+    const unsigned long long below = lane ? ((1ull << lane) - 1ull) : 0ull;
 
     long long out = FILL ? tile_off[T] : 0;
+    int my_n = 0;
     unsigned long long my_pairs = 0;
-    const int ci0 = max(0, TI + L.cell_mx - L.win_x), ci1 = min(L.cells_x - 1, TI + L.cell_mx + L.win_x);
-    const int cj0 = max(0, TJ + L.cell_my - L.win_y), cj1 = min(L.cells_y - 1, TJ + L.cell_my + L.win_y);
-    for (int ci = ci0; ci <= ci1; ++ci) {
-        for (int cj = cj0; cj <= cj1; ++cj) {
-            // conservative cell-rectangle test (midpoints of cell c along x lie
-            // in [x0 + ((c-m)*tile - 1/2) dx, x0 + ((c-m+1)*tile - 1/2) dx))
-            if (L.cells_x > 1) {
-                double cx0 = L.x0 + ((double)(ci - L.cell_mx) * L.tile_x - 0.5) * L.dx;
-                double cx1 = cx0 + (double)L.tile_x * L.dx;
-                double ex = fmax(fmax(cx0 - rx1, rx0 - cx1), 0.0);
-                double cy0 = 0.0, cy1 = 0.0, ey = 0.0;
-                if (L.cells_y > 1) {
-                    cy0 = L.y0 + ((double)(cj - L.cell_my) * L.tile_y - 0.5) * L.dy;
-                    cy1 = cy0 + (double)L.tile_y * L.dy;
-                    ey = fmax(fmax(cy0 - ry1, ry0 - cy1), 0.0);
-                }
-                if (ex * ex + ey * ey > L.rm2) continue;
-            } else if (L.cells_y > 1) {
-                double cy0 = L.y0 + ((double)(cj - L.cell_my) * L.tile_y - 0.5) * L.dy;
-                double cy1 = cy0 + (double)L.tile_y * L.dy;
-                double ey = fmax(fmax(cy0 - ry1, ry0 - cy1), 0.0);
-                if (ey * ey > L.rm2) continue;
+    auto scan = [&](int p0, int p1) {
+        for (int base = p0; base < p1; base += kListThreads * kListU) {
+            bool pass[kListU];
+            int kk[kListU];
+            float2 m[kListU];
+            // clamped loads (p1 > base): all slices in flight at once
+#pragma unroll
+            for (int u = 0; u < kListU; ++u) {
+                const int pc = min(base + u * kListThreads + tid, p1 - 1);
+                m[u] = smid[pc];
+                kk[u] = FILL ? sorted_ids[pc] : 0;
             }
-            const int c = ci * L.cells_y + cj;
-            const int p0 = cell_start[c], p1 = cell_start[c + 1];
-            for (int base = p0; base < p1; base += kListThreads) {
-                const int pos = base + threadIdx.x;
-                bool pass = false;
-                int k = 0;
-                float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (pos < p1) {
-                    k = sorted_ids[pos];
-                    g = geo[k];
-                    pass = point_rect_d2(g.x, g.y, rx0, rx1, ry0, ry1) <= L.rm2;
+#pragma unroll
+            for (int u = 0; u < kListU; ++u)
+                pass[u] = base + u * kListThreads + tid < p1 &&
+                          point_rect_d2(m[u].x, m[u].y, rx0, rx1, ry0, ry1) <= L.rm2;
+            if (FILL) {
+                unsigned long long bal[kListU];
+#pragma unroll
+                for (int u = 0; u < kListU; ++u) bal[u] = __ballot(pass[u]);
+                __syncthreads();   // the previous pass's counts have been read
+                if (lane == 0) {
+#pragma unroll
+                    for (int u = 0; u < kListU; ++u) s_cnt[u * kWaves + wave] = __popcll(bal[u]);
                 }
-                int total;
-                const int rank = block_rank(pass, s_wave, &total);
-                if (FILL) {
-                    if (pass) list[out + rank] = k;
-                } else if (pass) {
+                __syncthreads();
+                int run = 0;
+#pragma unroll
+                for (int u = 0; u < kListU; ++u) {
+                    int before = run;
+#pragma unroll
+                    for (int w = 0; w < kWaves; ++w) {
+                        const int c = s_cnt[u * kWaves + w];
+                        before += (w < wave) ? c : 0;
+                        run += c;
+                    }
+                    if (pass[u]) list[out + before + __popcll(bal[u] & below)] = kk[u];
+                }
+                out += run;
+            } else {
+#pragma unroll
+                for (int u = 0; u < kListU; ++u) {
+                    if (!pass[u]) continue;
+                    ++my_n;
                     for (int i = i_lo; i <= i_hi; ++i) {
                         const float x = axis_f32(L.x0, L.dx, i);
                         for (int j = j_lo; j <= j_hi; ++j)
-                            my_pairs += in_aperture_f32(g.x, g.y, x, axis_f32(L.y0, L.dy, j), L.r2);
+                            my_pairs += in_aperture_f32(m[u].x, m[u].y, x, axis_f32(L.y0, L.dy, j), L.r2);
                     }
                 }
-                out += total;
             }
+        }
+    };
+
+    const int ci0 = max(0, TI + L.cell_mx - L.win_x), ci1 = min(L.cells_x - 1, TI + L.cell_mx + L.win_x);
+    const int cj0 = max(0, TJ + L.cell_my - L.win_y), cj1 = min(L.cells_y - 1, TJ + L.cell_my + L.win_y);
+    for (int ci = ci0; ci <= ci1; ++ci) {
+        // first, last and number of passing cells of the column
+        if (tid == 0) {
+            s_lo = INT_MAX;
+            s_hi = INT_MIN;
+            s_n = 0;
+        }
+        __syncthreads();
+        {
+            int lo = INT_MAX, hi = INT_MIN, n = 0;
+            for (int cj = cj0 + tid; cj <= cj1; cj += kListThreads)
+                if (cell_in_reach(L, ci, cj, rx0, rx1, ry0, ry1)) {
+                    lo = min(lo, cj);
+                    hi = max(hi, cj);
+                    ++n;
+                }
+            if (n) {
+                atomicMin(&s_lo, lo);
+                atomicMax(&s_hi, hi);
+                atomicAdd(&s_n, n);
+            }
+        }
+        __syncthreads();
+        const int lo = s_lo, hi = s_hi, n = s_n;
+        __syncthreads();   // read by every thread before the next column resets them
+        if (n == 0) continue;
+        const int cb = ci * L.cells_y;
+        if (n == hi - lo + 1) {
+            scan(cell_start[cb + lo], cell_start[cb + hi + 1]);
+        } else {
+            for (int cj = lo; cj <= hi; ++cj)
+                if (cell_in_reach(L, ci, cj, rx0, rx1, ry0, ry1))
+                    scan(cell_start[cb + cj], cell_start[cb + cj + 1]);
         }
     }
     if (!FILL) {
+        if (my_n) atomicAdd((unsigned long long *)&s_count, (unsigned long long)my_n);
         if (my_pairs) atomicAdd(&s_pairs, my_pairs);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            tile_count[T] = out;
+        if (tid == 0) {
+            tile_count[T] = s_count;
             tile_pairs[T] = s_pairs;
         }
     }
+}
+
+// Sorted position -> midpoint, so that the list passes read candidates as
+// one coalesced stream instead of gathering geo[sorted_ids[p]].
+__global__ void gather_mid_kernel(int n, const int *__restrict__ sorted_ids,
+                                  const float4 *__restrict__ geo, float2 *__restrict__ smid)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 g = geo[sorted_ids[p]];
+    smid[p] = make_float2(g.x, g.y);
 }
 
 struct DevBuf {
@@ -312,6 +385,7 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
     const int gtr = (ntr + nb - 1) / nb;
 
     std::vector<int> h_start(p->n_cells + 1, 0);
+    DevBuf smid;   // sorted position -> midpoint, for the list passes only
     if (ntr > 0) {
         DevBuf keys, vals, skeys, tmp;
         PLAN_HIP(hipMalloc(&keys.p, sizeof(int) * ntr));
@@ -338,6 +412,12 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
         PLAN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, (int *)keys.p,
                                                     (int *)skeys.p, (int *)vals.p,
                                                     p->sorted_ids, ntr, 0, end_bit, st));
+        if (!p->all_traces) {
+            PLAN_HIP(hipMalloc(&smid.p, sizeof(float2) * ntr));
+            hipLaunchKernelGGL(gather_mid_kernel, dim3(gtr), dim3(nb), 0, st, ntr,
+                               (const int *)p->sorted_ids, (const float4 *)p->geo, (float2 *)smid.p);
+            if (check_launch("gather_mid_kernel") != PSKTM_OK) { plan_free(p); return PSKTM_E_LAUNCH; }
+        }
         // first/last sorted position of every non-empty cell
         DevBuf first, last;
         PLAN_HIP(hipMalloc(&first.p, sizeof(int) * p->n_cells));
@@ -374,7 +454,7 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
         PLAN_HIP(hipMalloc(&dcount.p, sizeof(long long) * nT));
         PLAN_HIP(hipMalloc(&dpairs.p, sizeof(unsigned long long) * nT));
         hipLaunchKernelGGL(tile_list_kernel<false>, dim3(nT), dim3(kListThreads), 0, st, L,
-                           p->geo, p->sorted_ids, p->cell_start, (long long *)dcount.p,
+                           (const float2 *)smid.p, p->sorted_ids, p->cell_start, (long long *)dcount.p,
                            (unsigned long long *)dpairs.p, (const long long *)nullptr,
                            (int *)nullptr);
         if (check_launch("tile_list_kernel<count>") != PSKTM_OK) { plan_free(p); return PSKTM_E_LAUNCH; }
@@ -410,7 +490,7 @@ static psktm_status plan_build(psktm_plan *p, const psktm_survey &s, hipStream_t
     if (!p->all_traces && p->list_len > 0) {
         PLAN_HIP(dalloc(p, &p->list, (size_t)p->list_len));
         hipLaunchKernelGGL(tile_list_kernel<true>, dim3(nT), dim3(kListThreads), 0, st, L,
-                           p->geo, p->sorted_ids, p->cell_start, (long long *)nullptr,
+                           (const float2 *)smid.p, p->sorted_ids, p->cell_start, (long long *)nullptr,
                            (unsigned long long *)nullptr, (const long long *)p->tile_off,
                            p->list);
         if (check_launch("tile_list_kernel<fill>") != PSKTM_OK) { plan_free(p); return PSKTM_E_LAUNCH; }
