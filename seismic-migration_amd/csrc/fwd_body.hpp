@@ -168,7 +168,7 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
 // 16-byte stores.
 template <int NTH>
 __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, int G,
-                                               unsigned *__restrict__ s_half)
+                                               unsigned *__restrict__ s_half, int k_first)
 {
     constexpr int W = NTH / 64;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -181,7 +181,8 @@ __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, in
     // one trace per wave at a time: its row set-up once, then its chunks
     for (int g = wave; g < G; g += W) {
         const long long q = s + g;
-        const int k = a.all_traces ? (int)q : a.list[q];
+        // the wave's first trace of the stage comes loaded one stage ahead
+        const int k = g == wave ? k_first : (a.all_traces ? (int)q : a.list[q]);
         const uintptr_t ra = (uintptr_t)a.traces + (size_t)k * a.ns * 2;
         const int sb = (int)(ra & 15u);
         const int n_chunks = (sb + a.ns * 2) / 16 + 1;
@@ -384,6 +385,18 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
     const long long s0 = l0 + (l1 - l0) * split / a.n_split;
     const long long s1 = l0 + (l1 - l0) * (split + 1) / a.n_split;
     const int G = a.G;
+    // Half pairs: list entries are loaded one stage ahead, during the
+    // previous stage's compute: the distance-term lane's trace (kp, lane
+    // tid < G*TB: trace tid / TB) and the fill's first trace of each wave
+    // (kf, wave w: trace w). The stage's set-up then waits for one load
+    // (geometry, rows) instead of two in a row. (The FP32-pair forms keep
+    // the in-stage loads: the live entry would cost them a wave per SIMD.)
+    const int wave_u = __builtin_amdgcn_readfirstlane(tid >> 6);
+    auto list_at = [&](long long q) -> int {
+        return q < s1 ? (a.all_traces ? (int)q : a.list[q]) : 0;
+    };
+    int kp = (HP && tid < G * TB) ? list_at(s0 + tid / TB) : 0;
+    int kf = HP ? list_at(s0 + wave_u) : 0;
     auto stages = [&](auto clamp_tag) {
     constexpr bool CL = decltype(clamp_tag)::value;
     for (long long s = s0; s < s1; s += G) {
@@ -394,7 +407,7 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             float2 ab = make_float2(0.f, 0.f);
             bool m = false;
             if (q < s1) {
-                const int k = a.all_traces ? (int)q : a.list[q];
+                const int k = HP ? kp : (a.all_traces ? (int)q : a.list[q]);
                 const int bi = b / a.tile_y, bj = b % a.tile_y;
                 const int i = i0 + bi, jj = j0 + bj;
                 if (bi < a.tile_x && i < a.nx && jj < a.ny)
@@ -407,18 +420,17 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                 s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
                 // half pairs hold half the stored values: twice the trace's
                 // scale (exact) restores them in the accumulate
-                if (HP) {
-                    const int k = q < s1 ? (a.all_traces ? (int)q : a.list[q]) : 0;
-                    s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[k] : 0.f);
-                }
+                if (HP) s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[kp] : 0.f);
             }
         }
         if (HP)
             fill_half_rows<NTH>(a, s, (int)((s1 - s) < (long long)a.G ? (s1 - s) : (long long)a.G),
-                                reinterpret_cast<unsigned *>(s_pairs));
+                                reinterpret_cast<unsigned *>(s_pairs), kf);
         else
             fill_stage<NTH>(a, s, s1, s_pairs);
         __syncthreads();
+        if (HP && tid < G * TB) kp = list_at(s + G + tid / TB);
+        if (HP) kf = list_at(s + G + wave_u);
         // ---- compute --------------------------------------------------------
         // traces staged in this pass (a 32-bit scalar bound: the 64-bit test
         // s + g < s1 would cost two VALU instructions per trace)
