@@ -8,6 +8,13 @@ choice with all timings in a JSON cache keyed by a workload signature. With a
 cache entry present it returns the cached choice without timing anything, so
 the choice is deterministic for a fixed cache. Ties go to the earlier
 candidate.
+
+Trials on part of a workload can rank candidates within their run-to-run
+spread (on the 64-gather batch, nine candidates timed on a quarter of the
+gathers came within 17%, and the quickest there ran 15% slower on the whole
+batch than the runner-up). ``refine_fn`` then times the ``refine_top``
+fastest trial candidates on the whole workload and the choice is the
+quickest of those.
 """
 from __future__ import annotations
 
@@ -51,9 +58,13 @@ class Autotuner:
         return tuple(e["choice"]) if e else None
 
     def choose(self, key: str, candidates: Iterable[Candidate],
-               time_fn: Callable[[Candidate], Optional[float]], retune: bool = False) -> Candidate:
+               time_fn: Callable[[Candidate], Optional[float]], retune: bool = False,
+               refine_fn: Optional[Callable[[Candidate], Optional[float]]] = None,
+               refine_top: int = 3) -> Candidate:
         """Fastest candidate for ``key``; ``time_fn`` returns seconds, or None
-        when a candidate is not supported for the workload."""
+        when a candidate is not supported for the workload. With ``refine_fn``
+        (seconds on the whole workload), the ``refine_top`` fastest candidates
+        of ``time_fn`` are timed again by it and the fastest there is chosen."""
         if not retune:
             hit = self.lookup(key)
             if hit is not None:
@@ -68,8 +79,24 @@ class Autotuner:
                 best, best_t = c, t
         if best is None:
             raise RuntimeError(f"no candidate ran for {key}")
-        self.cache[key] = {"choice": list(best), "seconds": best_t, "timings": timings,
-                           "tuned_at": time.strftime("%Y-%m-%dT%H:%M:%S")}
+        entry = {"choice": list(best), "seconds": best_t, "timings": timings}
+        if refine_fn is not None and refine_top > 1:
+            ranked = sorted((t, i, c) for i, (c, t) in
+                            enumerate((tuple(int(v) for v in k.split(",")), t)
+                                      for k, t in timings.items()) if t is not None)
+            refined = {}
+            best, best_t = None, math.inf
+            for _, _, c in ranked[:refine_top]:
+                t = refine_fn(c)
+                refined[",".join(map(str, c))] = t
+                if t is not None and t < best_t:
+                    best, best_t = c, t
+            if best is None:
+                raise RuntimeError(f"no candidate ran on the whole workload for {key}")
+            entry = {"choice": list(best), "seconds": best_t, "timings": timings,
+                     "refined": refined}
+        entry["tuned_at"] = time.strftime("%Y-%m-%dT%H:%M:%S")
+        self.cache[key] = entry
         self.save()
         return best
 

@@ -39,3 +39,22 @@ def test_no_runnable_candidate_raises(tmp_path):
 def test_candidates_are_valid_tile_sizes():
     for c in list(CANDIDATES_3D) + list(CANDIDATES_2D):
         assert c[0] * c[1] in (8, 16) and 1 <= c[2] <= 8
+
+
+def test_refine_times_the_best_trials_on_the_whole_workload(tmp_path):
+    # trials rank (4,4,2) < (8,2,4) < (2,8,4) < (4,4,4); on the whole workload
+    # the third of them is the quickest, the fourth is never timed again
+    trial = {(4, 4, 4): 2.0, (4, 4, 2): 1.0, (4, 4, 8): None, (8, 2, 4): 1.1, (2, 8, 4): 1.2}
+    whole = {(4, 4, 2): 5.0, (8, 2, 4): None, (2, 8, 4): 4.0}
+    refined = []
+    t = Autotuner(str(tmp_path / "c.json"))
+    got = t.choose("k", trial, lambda c: trial[c],
+                   refine_fn=lambda c: refined.append(c) or whole[c], refine_top=3)
+    assert got == (2, 8, 4)
+    assert refined == [(4, 4, 2), (8, 2, 4), (2, 8, 4)]
+    saved = json.load(open(tmp_path / "c.json"))["k"]
+    assert saved["choice"] == [2, 8, 4] and saved["seconds"] == 4.0
+    assert saved["refined"] == {"4,4,2": 5.0, "8,2,4": None, "2,8,4": 4.0}
+    # the cached choice comes back without timing anything
+    assert Autotuner(str(tmp_path / "c.json")).choose(
+        "k", trial, lambda c: 1 / 0, refine_fn=lambda c: 1 / 0) == (2, 8, 4)
