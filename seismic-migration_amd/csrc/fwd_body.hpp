@@ -181,8 +181,9 @@ __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, in
     // one trace per wave at a time: its row set-up once, then its chunks
     for (int g = wave; g < G; g += W) {
         const long long q = s + g;
-        // the wave's first trace of the stage comes loaded one stage ahead
-        const int k = g == wave ? k_first : (a.all_traces ? (int)q : a.list[q]);
+        // the wave's first trace of the stage, when loaded one stage ahead
+        // (k_first >= 0)
+        const int k = (g == wave && k_first >= 0) ? k_first : (a.all_traces ? (int)q : a.list[q]);
         const uintptr_t ra = (uintptr_t)a.traces + (size_t)k * a.ns * 2;
         const int sb = (int)(ra & 15u);
         const int n_chunks = (sb + a.ns * 2) / 16 + 1;
@@ -296,9 +297,11 @@ __device__ __forceinline__ bool pair_terms(float4 ge, int i, int j, float x0, fl
 // One output tile (T, tau pass `chunk`) of a migration: the whole forward
 // algorithm described in migrate.hip. GN: gained traces (sample n carries
 // sqrt(n dt)), accumulated without the per-update sqrt(t) weight; a template
-// parameter so that each form keeps its own register allocation. Shared by the per-plan kernel and the
-// persistent batch kernel; the caller provides the LDS arrays.
-template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP = false>
+// parameter so that each form keeps its own register allocation. PF (half
+// pairs only): each stage's list entries are loaded one stage ahead. Shared
+// by the per-plan kernel and the persistent batch kernel; the caller
+// provides the LDS arrays.
+template <int NTH, int TB, int TL, bool V3, bool KH, bool GN, bool HP = false, bool PF = false>
 __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const int chunk,
                                          const int split,
                                          float2 *__restrict__ s_pairs,
@@ -385,18 +388,20 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
     const long long s0 = l0 + (l1 - l0) * split / a.n_split;
     const long long s1 = l0 + (l1 - l0) * (split + 1) / a.n_split;
     const int G = a.G;
-    // Half pairs: list entries are loaded one stage ahead, during the
-    // previous stage's compute: the distance-term lane's trace (kp, lane
-    // tid < G*TB: trace tid / TB) and the fill's first trace of each wave
-    // (kf, wave w: trace w). The stage's set-up then waits for one load
-    // (geometry, rows) instead of two in a row. (The FP32-pair forms keep
-    // the in-stage loads: the live entry would cost them a wave per SIMD.)
+    // PF: list entries are loaded one stage ahead, during the previous
+    // stage's compute: the distance-term lane's trace (kp, lane tid < G*TB:
+    // trace tid / TB) and the fill's first trace of each wave (kf, wave w:
+    // trace w). The stage's set-up then waits for one load (geometry, rows)
+    // instead of two in a row. Only the per-plan half-pair kernel, which
+    // caps its registers, takes it: the live entries cost the FP32-pair
+    // forms a wave per SIMD, and the uncapped batch kernel 7-10 VGPRs.
     const int wave_u = __builtin_amdgcn_readfirstlane(tid >> 6);
     auto list_at = [&](long long q) -> int {
         return q < s1 ? (a.all_traces ? (int)q : a.list[q]) : 0;
     };
-    int kp = (HP && tid < G * TB) ? list_at(s0 + tid / TB) : 0;
-    int kf = HP ? list_at(s0 + wave_u) : 0;
+    static_assert(!PF || HP, "the stage look-ahead is for half pairs");
+    int kp = (PF && tid < G * TB) ? list_at(s0 + tid / TB) : 0;
+    int kf = PF ? list_at(s0 + wave_u) : -1;
     auto stages = [&](auto clamp_tag) {
     constexpr bool CL = decltype(clamp_tag)::value;
     for (long long s = s0; s < s1; s += G) {
@@ -407,7 +412,7 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
             float2 ab = make_float2(0.f, 0.f);
             bool m = false;
             if (q < s1) {
-                const int k = HP ? kp : (a.all_traces ? (int)q : a.list[q]);
+                const int k = PF ? kp : (a.all_traces ? (int)q : a.list[q]);
                 const int bi = b / a.tile_y, bj = b % a.tile_y;
                 const int i = i0 + bi, jj = j0 + bj;
                 if (bi < a.tile_x && i < a.nx && jj < a.ny)
@@ -420,7 +425,10 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                 s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
                 // half pairs hold half the stored values: twice the trace's
                 // scale (exact) restores them in the accumulate
-                if (HP) s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[kp] : 0.f);
+                if (HP) {
+                    const int k = !(q < s1) ? 0 : PF ? kp : (a.all_traces ? (int)q : a.list[q]);
+                    s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[k] : 0.f);
+                }
             }
         }
         if (HP)
@@ -429,8 +437,8 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
         else
             fill_stage<NTH>(a, s, s1, s_pairs);
         __syncthreads();
-        if (HP && tid < G * TB) kp = list_at(s + G + tid / TB);
-        if (HP) kf = list_at(s + G + wave_u);
+        if (PF && tid < G * TB) kp = list_at(s + G + tid / TB);
+        if (PF) kf = list_at(s + G + wave_u);
         // ---- compute --------------------------------------------------------
         // traces staged in this pass (a 32-bit scalar bound: the 64-bit test
         // s + g < s1 would cost two VALU instructions per trace)

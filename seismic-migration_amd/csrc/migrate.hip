@@ -93,8 +93,8 @@ void migrate_fwd_kernel(const FwdArgs a)
     // block -> (tile in work order, tau pass, split); splits of a tile adjacent
     const int split = blockIdx.x % a.n_split;
     const int rest = blockIdx.x / a.n_split;
-    fwd_tile<NTH, TB, TL, V3, KH, GN, HP>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks,
-                                          split, s_pairs, s_ab, s_mask);
+    fwd_tile<NTH, TB, TL, V3, KH, GN, HP, HP>(a, a.tile_order[rest / a.n_chunks], rest % a.n_chunks,
+                                              split, s_pairs, s_ab, s_mask);
 }
 
 // Exact fixed-point accumulation of the adjoint. A lane's merged
