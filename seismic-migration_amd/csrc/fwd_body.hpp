@@ -166,7 +166,7 @@ __device__ __forceinline__ void fill_stage(const FwdArgs &a, long long s, long l
 // -32752 (both valid quantised samples) would difference to -inf. Rows start
 // 16-byte aligned, so a chunk whose first entry is a multiple of 4 writes two
 // 16-byte stores.
-template <int NTH>
+template <int NTH, bool PF>
 __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, int G,
                                                unsigned *__restrict__ s_half, int k_first)
 {
@@ -181,9 +181,8 @@ __device__ __forceinline__ void fill_half_rows(const FwdArgs &a, long long s, in
     // one trace per wave at a time: its row set-up once, then its chunks
     for (int g = wave; g < G; g += W) {
         const long long q = s + g;
-        // the wave's first trace of the stage, when loaded one stage ahead
-        // (k_first >= 0)
-        const int k = (g == wave && k_first >= 0) ? k_first : (a.all_traces ? (int)q : a.list[q]);
+        // PF: the wave's first trace of the stage comes loaded one stage ahead
+        const int k = (PF && g == wave) ? k_first : (a.all_traces ? (int)q : a.list[q]);
         const uintptr_t ra = (uintptr_t)a.traces + (size_t)k * a.ns * 2;
         const int sb = (int)(ra & 15u);
         const int n_chunks = (sb + a.ns * 2) / 16 + 1;
@@ -401,7 +400,7 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
     };
     static_assert(!PF || HP, "the stage look-ahead is for half pairs");
     int kp = (PF && tid < G * TB) ? list_at(s0 + tid / TB) : 0;
-    int kf = PF ? list_at(s0 + wave_u) : -1;
+    int kf = PF ? list_at(s0 + wave_u) : 0;
     auto stages = [&](auto clamp_tag) {
     constexpr bool CL = decltype(clamp_tag)::value;
     for (long long s = s0; s < s1; s += G) {
@@ -425,14 +424,16 @@ __device__ __forceinline__ void fwd_tile(const FwdArgs &a, const int T, const in
                 s_mask[g] = (unsigned)((bal >> shift) & ((1ull << TB) - 1ull));
                 // half pairs hold half the stored values: twice the trace's
                 // scale (exact) restores them in the accumulate
-                if (HP) {
-                    const int k = !(q < s1) ? 0 : PF ? kp : (a.all_traces ? (int)q : a.list[q]);
+                if constexpr (PF) {
+                    s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[kp] : 0.f);
+                } else if (HP) {
+                    const int k = q < s1 ? (a.all_traces ? (int)q : a.list[q]) : 0;
                     s_mask[kGMax + g] = __float_as_uint(q < s1 ? 2.f * a.scale[k] : 0.f);
                 }
             }
         }
         if (HP)
-            fill_half_rows<NTH>(a, s, (int)((s1 - s) < (long long)a.G ? (s1 - s) : (long long)a.G),
+            fill_half_rows<NTH, PF>(a, s, (int)((s1 - s) < (long long)a.G ? (s1 - s) : (long long)a.G),
                                 reinterpret_cast<unsigned *>(s_pairs), kf);
         else
             fill_stage<NTH>(a, s, s1, s_pairs);
